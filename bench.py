@@ -1,0 +1,188 @@
+"""Benchmark: retro-edges per second of the plain k-cop solve on one MI355X.
+
+Workload (default): Erdos-Renyi G(200, 0.03), first connected seed >= 1
+(tests/golden/matrices/er200_p0.03_s1.txt), k = 3 cops, full information
+(p = 1): C(202, 3) = 1,353,400 configurations, 5.41e8 states. A step is one
+full solve with the tables resident in HBM: initializeCaptures + mainLoop
+(every depth level until the fixed point). The device tables are allocated
+once before the timed region.
+
+Unit of work: a retro-edge, one predecessor relaxation of the push
+formulation, E_retro = E_cop + E_rob (won robber-turn states x |pred(C')| +
+won cop-turn states x in-degree(v), depth <= 125). value = E_retro * steps /
+elapsed, in GTEPS.
+
+Also reported:
+  roofline     the dominant kernel's algorithmic bytes (0.5 B per cop
+               retro-edge for cop_push: 2 bits read + 2 bits written per
+               relaxation; 2 B per robber retro-edge for robber_step) over its
+               own HIP-event time, against 8 TB/s.
+  cpu_baseline the reference solver (oracle/_ref, built from the reference's
+               sources) on a bounded sample -- the same graph at k = 2 -- on
+               one host core, as GTEPS; plus the multithreaded oracle port on
+               the full workload when --cpu-port is given.
+"""
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK = 8.0e12
+WORKLOADS = {
+    "er200_k3": ("er200_p0.03_s1", 3),
+    "er200_k2": ("er200_p0.03_s1", 2),
+    "planar200_k4": ("planar200_e400_s1", 4),
+    "sy_all_k2": ("scotlandyard-all", 2),
+}
+# algorithmic bytes per unit of work, by kernel
+BYTES_PER_UNIT = {"cop_push": ("e_cop", 0.5), "robber_step": ("e_rob", 2.0)}
+
+
+def matrix_path(stem):
+    return os.path.join(REPO, "tests", "golden", "matrices", stem + ".txt")
+
+
+def cpu_reference_sample(stem, k, e_retro):
+    """Reference binary (single thread) on a bounded sample; GTEPS from its Main Loop time."""
+    exe = os.path.join(REPO, "oracle", "_ref", "k_cops_visibility")
+    if not os.path.exists(exe):
+        return None
+    work = tempfile.mkdtemp(prefix="crbench_")
+    try:
+        proc = subprocess.run([exe, matrix_path(stem), str(k), "1"], cwd=work, capture_output=True,
+                              text=True, timeout=600)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    m = re.search(r"Main Loop\s*-*=>\s*([\d.]+)\s*s", proc.stdout)
+    if not m:
+        return None
+    secs = float(m.group(1))
+    return {"value": e_retro / secs / 1e9, "unit": "GTEPS", "cores": 1, "kind": "reference",
+            "sample": f"reference k_cops_visibility, same graph at k={k} ({stem}), main loop "
+                      f"{secs:.2f} s for {e_retro} retro-edges"}
+
+
+def cpu_port(stem, k, threads):
+    from oracle import binding as oracle
+    n, adj = oracle.load_matrix(matrix_path(stem))
+    t0 = time.perf_counter()
+    states, _, e_cop, e_rob = oracle.v1_levelsync(adj, k, threads)
+    dt = time.perf_counter() - t0
+    return {"value": (e_cop + e_rob) / dt / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"multithreaded level-synchronous oracle, full workload, {dt:.2f} s"}, states
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="er200_k3", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-port", type=int, default=0, help="also time the oracle port on N threads")
+    ap.add_argument("--check", action="store_true", help="compare the table with the oracle port")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this solver runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "design-only)"}))
+        return 2
+
+    import torch
+
+    import copsandrobbers_amd as cra
+
+    stem, k = WORKLOADS[args.workload]
+    path = matrix_path(stem)
+    s = cra.Solver(0)
+    s.load_graph_file(path, k, 1)
+    s.build_aux_graph()
+    info = s.info()
+    for _ in range(args.warmup):
+        s.initialize_captures()
+        s.main_loop()
+    e_cop, e_rob, won = s.retro_edges()
+    e_retro = e_cop + e_rob
+    s.set_kernel_timing(True)
+    kt_total = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.initialize_captures()
+        passes = s.main_loop()
+        for name, (ms, launches) in s.kernel_times().items():
+            a = kt_total.setdefault(name, [0.0, 0])
+            a[0] += ms
+            a[1] += launches
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    verdict = s.find_final_result()
+    stats = s.stats()
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = e_retro * args.steps / elapsed / 1e9
+
+    units = {"e_cop": e_cop, "e_rob": e_rob}
+    dominant = max(kt_total, key=lambda n: kt_total[n][0])
+    rl = None
+    if dominant in BYTES_PER_UNIT:
+        unit_name, per = BYTES_PER_UNIT[dominant]
+        ms, launches = kt_total[dominant]
+        bytes_per_launch = units[unit_name] * per * args.steps / launches
+        avg_s = ms / 1e3 / launches
+        achieved = bytes_per_launch / avg_s
+        rl = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+              "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dominant,
+              "avg_launch_us": avg_s * 1e6, "launches_per_step": launches // args.steps}
+    b_alg = 2 * e_rob + 0.5 * e_cop + won * 1.25 + info.num_states * (0.25 + 1) + info.num_states / 2
+    out = {
+        "metric": "retro-edges/s (GTEPS)",
+        "value": value,
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/u64 bitsets (integer)",
+        "data": "synthetic (seeded graph generator, committed matrix)",
+        "config": {"workload": args.workload, "graph": stem, "n": info.n, "k": k, "p": 1,
+                   "configs": info.config_count, "states": info.num_states,
+                   "e_cop": e_cop, "e_rob": e_rob, "won_states": won, "levels": stats.n_levels,
+                   "passes": len(passes), "verdict": "WIN" if verdict else "LOSS"},
+        "roofline": rl,
+        "solve_alg_bytes_per_s": b_alg / (elapsed / args.steps),
+        "kernels_ms_per_step": {n: v[0] / args.steps for n, v in kt_total.items()},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        # bounded sample: the reference on the same graph with one cop fewer
+        sk = max(1, k - 1)
+        with cra.Solver(0) as t:
+            t.load_graph_file(path, sk, 1)
+            t.build_aux_graph()
+            t.initialize_captures()
+            t.main_loop()
+            se = sum(t.retro_edges()[:2])
+        out["cpu_baseline"] = cpu_reference_sample(stem, sk, se)
+    if args.cpu_port:
+        port, states = cpu_port(stem, k, args.cpu_port)
+        out["cpu_port"] = port
+        if args.check:
+            import numpy as np
+            out["check_vs_port"] = bool(np.array_equal(states, s.states()))
+    s.close()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

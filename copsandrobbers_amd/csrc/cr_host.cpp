@@ -1,0 +1,228 @@
+// cr_host.cpp -- host-side pieces of the solver (see cr_host.h).
+#include "cr_host.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <thread>
+
+#include "crsolve.h"
+
+namespace cr {
+
+int32_t parse_matrix(const uint8_t* text, uint64_t len, uint8_t* adj, int32_t cap) {
+    if (text == nullptr || len == 0) return 0;  // fileio.cpp:12 rejects empty files
+    uint64_t width = 0;  // Graph.cpp:27-30: width of the first line
+    while (width < len && text[width] != '\n' && text[width] != '\r' && text[width] != '-') ++width;
+    if (width == 0 || width > 0x7fffffff) return 0;
+    const int32_t n = static_cast<int32_t>(width);
+    if (adj == nullptr || n > cap) return n;
+    std::memset(adj, 0, static_cast<size_t>(n) * n);
+    int32_t row = 0, col = 0;
+    for (uint64_t i = 0; i < len; ++i) {
+        const uint8_t ch = text[i];
+        if (ch == '-') break;
+        if (ch == '0' || ch == '1') {
+            if (row < n && col < n) adj[static_cast<size_t>(row) * n + col] = (ch == '1');
+            ++col;
+        } else if (ch == '\n' && col > 0) {
+            ++row;
+            col = 0;
+        }
+    }
+    return n;
+}
+
+void Adjacency::build(int32_t n_, const uint8_t* adj) {
+    n = n_;
+    matrix.assign(adj, adj + static_cast<size_t>(n) * n);
+    out.assign(n, {});
+    in.assign(n, {});
+    for (int32_t u = 0; u < n; ++u)
+        for (int32_t v = 0; v < n; ++v)
+            if (adj[static_cast<size_t>(u) * n + v]) {
+                out[u].push_back(static_cast<uint8_t>(v));
+                in[v].push_back(static_cast<uint8_t>(u));
+            }
+    max_out = max_in = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        max_out = std::max<int32_t>(max_out, static_cast<int32_t>(out[v].size()));
+        max_in = std::max<int32_t>(max_in, static_cast<int32_t>(in[v].size()));
+    }
+}
+
+uint64_t binom(int64_t a, int64_t b) {
+    if (b < 0 || a < b) return 0;
+    if (b > a - b) b = a - b;
+    unsigned __int128 acc = 1;
+    for (int64_t i = 1; i <= b; ++i) acc = acc * static_cast<uint64_t>(a - b + i) / static_cast<uint64_t>(i);
+    return acc > UINT64_MAX ? UINT64_MAX : static_cast<uint64_t>(acc);
+}
+
+uint64_t config_count(int32_t n, int32_t k) {
+    if (k <= 0 || k > 256 || n <= 0) return 0;
+    return binom(static_cast<int64_t>(n) + k - 1, k);
+}
+
+// Non-decreasing tuples c_0 <= ... <= c_{k-1} over [0, n) are the strictly
+// increasing subsets e_i = c_i + i of [0, M), M = n+k-1, in the same
+// lexicographic order. The subsets after e number
+// sum_i C(M-1-e_i, k-i) (combinatorial number system), so
+// rank = C(M, k) - 1 - that sum.
+uint64_t rank_tuple(const uint8_t* c, int32_t n, int32_t k) {
+    const int64_t M = static_cast<int64_t>(n) + k - 1;
+    uint64_t tail = 0;
+    for (int32_t i = 0; i < k; ++i) tail += binom(M - 1 - (static_cast<int64_t>(c[i]) + i), k - i);
+    return binom(M, k) - 1 - tail;
+}
+
+void unrank_tuple(uint64_t rank, int32_t n, int32_t k, uint8_t* out) {
+    const int64_t M = static_cast<int64_t>(n) + k - 1;
+    uint64_t s = binom(M, k) - 1 - rank;
+    int64_t xmax = M - 1;
+    for (int32_t i = 0; i < k; ++i) {
+        const int32_t j = k - i;
+        int64_t lo = 0, hi = xmax;  // largest x <= xmax with C(x, j) <= s
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) / 2;
+            if (binom(mid, j) <= s) lo = mid; else hi = mid - 1;
+        }
+        s -= binom(lo, j);
+        out[i] = static_cast<uint8_t>((M - 1 - lo) - i);
+        xmax = lo - 1;
+    }
+}
+
+void team_moves(const std::vector<std::vector<uint8_t>>& nbr, const uint8_t* cops, int32_t k,
+                int32_t n, bool stay, std::vector<uint64_t>& out) {
+    out.clear();
+    std::vector<std::vector<uint8_t>> opt(k);
+    for (int32_t i = 0; i < k; ++i) {
+        if (stay) opt[i].push_back(cops[i]);
+        opt[i].insert(opt[i].end(), nbr[cops[i]].begin(), nbr[cops[i]].end());
+        if (opt[i].empty()) return;
+    }
+    std::vector<int32_t> digit(k, 0);
+    uint8_t t[CR_MAX_K > 16 ? CR_MAX_K : 16];
+    for (;;) {
+        for (int32_t i = 0; i < k; ++i) t[i] = opt[i][digit[i]];
+        std::sort(t, t + k);
+        out.push_back(rank_tuple(t, n, k));
+        int32_t j = k - 1;
+        while (j >= 0 && ++digit[j] == static_cast<int32_t>(opt[j].size())) digit[j--] = 0;
+        if (j < 0) break;
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+}
+
+std::string cache_key(const std::string& algo, const std::string& graph_path, int32_t k, int32_t p,
+                      bool cop_stay, bool robber_stay) {
+    const std::string stem = std::filesystem::path(graph_path).stem().string();
+    const std::string flags = std::string(cop_stay ? "1" : "0") + (robber_stay ? "1" : "0");
+    return "cache/" + algo + "__" + stem + "__" + std::to_string(k) + "-cops__" + std::to_string(p) +
+           "-vis__" + flags + "__auxgraph.bin";
+}
+
+namespace {
+
+unsigned worker_count() {
+    unsigned h = std::thread::hardware_concurrency();
+    if (h == 0) h = 1;
+    return std::min(h, 16u);
+}
+
+template <typename Fn>
+void parallel_ranges(uint64_t total, Fn fn) {
+    const unsigned nt = worker_count();
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t lo = total * t / nt, hi = total * (t + 1) / nt;
+        th.emplace_back([=] { fn(lo, hi); });
+    }
+    for (auto& x : th) x.join();
+}
+
+bool put(std::FILE* f, const void* p, size_t bytes) {
+    return bytes == 0 || std::fwrite(p, 1, bytes, f) == bytes;
+}
+
+}  // namespace
+
+int write_aux_blob(const std::string& path, const Adjacency& g, int32_t k, int32_t columns,
+                   bool cop_stay, bool robber_stay, uint64_t num_states,
+                   const std::function<int(uint64_t, uint64_t, uint8_t*)>& fetch_states) {
+    const uint64_t C = config_count(g.n, k);
+    const uint64_t N = static_cast<uint64_t>(g.n);
+    // pass 1: |T(C)| per configuration -> heads
+    std::vector<uint64_t> heads(C + 1, 0);
+    parallel_ranges(C, [&](uint64_t lo, uint64_t hi) {
+        std::vector<uint64_t> mv;
+        uint8_t cops[CR_MAX_K];
+        for (uint64_t c = lo; c < hi; ++c) {
+            unrank_tuple(c, g.n, k, cops);
+            team_moves(g.out, cops, k, g.n, cop_stay, mv);
+            heads[c + 1] = mv.size();
+        }
+    });
+    for (uint64_t c = 0; c < C; ++c) heads[c + 1] += heads[c];
+    const uint64_t ntrans = heads[C];
+    const uint64_t header_bytes = 12 * sizeof(uint64_t);
+    const uint64_t s1 = header_bytes;
+    const uint64_t s2 = s1 + C * static_cast<uint64_t>(k);
+    const uint64_t s3 = s2 + (C + 1) * sizeof(uint64_t);
+    const uint64_t s4 = s3 + ntrans * sizeof(uint64_t);
+    const uint64_t header[12] = {static_cast<uint64_t>(k), N, static_cast<uint64_t>(columns), C,
+                                 num_states, 1, cop_stay ? 1u : 0u, robber_stay ? 1u : 0u,
+                                 s1, s2, s3, s4};
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return CR_ERR_IO;
+    bool ok = put(f, header, sizeof header);
+    {  // configs, in chunks
+        const uint64_t chunk = 1 << 20;
+        std::vector<uint8_t> buf;
+        for (uint64_t c0 = 0; ok && c0 < C; c0 += chunk) {
+            const uint64_t c1 = std::min(C, c0 + chunk);
+            buf.resize((c1 - c0) * k);
+            parallel_ranges(c1 - c0, [&](uint64_t lo, uint64_t hi) {
+                for (uint64_t c = lo; c < hi; ++c) unrank_tuple(c0 + c, g.n, k, &buf[c * k]);
+            });
+            ok = put(f, buf.data(), buf.size());
+        }
+    }
+    ok = ok && put(f, heads.data(), heads.size() * sizeof(uint64_t));
+    {  // transitions (nextId * N), in chunks of configurations
+        const uint64_t chunk = 1 << 16;
+        std::vector<uint64_t> buf;
+        for (uint64_t c0 = 0; ok && c0 < C; c0 += chunk) {
+            const uint64_t c1 = std::min(C, c0 + chunk);
+            buf.assign(heads[c1] - heads[c0], 0);
+            parallel_ranges(c1 - c0, [&](uint64_t lo, uint64_t hi) {
+                std::vector<uint64_t> mv;
+                uint8_t cops[CR_MAX_K];
+                for (uint64_t c = c0 + lo; c < c0 + hi; ++c) {
+                    unrank_tuple(c, g.n, k, cops);
+                    team_moves(g.out, cops, k, g.n, cop_stay, mv);
+                    uint64_t* dst = &buf[heads[c] - heads[c0]];
+                    for (size_t i = 0; i < mv.size(); ++i) dst[i] = mv[i] * N;
+                }
+            });
+            ok = put(f, buf.data(), buf.size() * sizeof(uint64_t));
+        }
+    }
+    {  // states, fetched from the device in chunks
+        const uint64_t chunk = 64ull << 20;
+        std::vector<uint8_t> buf;
+        for (uint64_t o = 0; ok && o < num_states; o += chunk) {
+            const uint64_t cnt = std::min(chunk, num_states - o);
+            buf.resize(cnt);
+            if (fetch_states(o, cnt, buf.data()) != 0) ok = false;
+            ok = ok && put(f, buf.data(), cnt);
+        }
+    }
+    if (std::fclose(f) != 0) ok = false;
+    return ok ? 0 : CR_ERR_IO;
+}
+
+}  // namespace cr

@@ -1,0 +1,1271 @@
+// crsolve.hip -- MI355X (gfx950) k-cop retrograde solver behind the C ABI of
+// include/crsolve.h.
+//
+// State space (reference: lib/include/AuxGraph.h:59-113): a configuration C is
+// a sorted k-tuple of cop vertices, identified by its lexicographic rank cId
+// (AuxGraph.h:117-177 enumeration order, computed here in closed form); a
+// state is (cId, r, turn).
+//
+// Device layout for the plain game (p = 1, k_cops_visibility.cpp with two
+// columns):
+//   * bit planes, one row of W = ceil(N/64) 64-bit words per configuration,
+//     bit r = robber vertex r:
+//       won0 / won1  states already won with the cops / the robber to move
+//       fr1          robber-turn states won at the current depth (frontier)
+//       nx0          cop-turn candidates of the next depth (atomicOr target)
+//   * table: the reference's AoS DataItem bytes, index (cId*N + r)*2 + turn,
+//     written once per state when it is won.
+//   * row lists of configurations holding new states (compacted with
+//     wavefront ballots and one atomic per wavefront).
+//
+// One depth level d -> d+1 (level-synchronous retrograde analysis; equal to
+// the reference's pass sweep, whose pass i marks cop states of depth 2i-2 and
+// 2i-1 and robber states of depth 2i-1 and 2i):
+//   k_cop_push      every configuration C' with new robber-turn states at
+//                   depth d ORs its robber mask into nx0 of each predecessor C
+//                   (each cop steps back along an in-edge; the product is
+//                   enumerated, sorted and ranked in registers). Cop turn =
+//                   "exists a move to a won state", depth = min + 1
+//                   (k_cops_visibility.cpp:160-197).
+//   k_robber_step   for every configuration with new cop-turn states at depth
+//                   d, robber r is won iff every move of r lands on a won
+//                   cop-turn state: out_mask[r] subset of won0 row; depth =
+//                   max + 1 (k_cops_visibility.cpp:235-288 with p = 1).
+//   k_cop_finalize  nx0 & ~won0 becomes the cop-turn states of depth d+1.
+// States whose depth would reach 127 stay unmarked (the 7-bit cap, :189,
+// :220, :280), so levels run d = 0..125.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cr_host.h"
+#include "crsolve.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr unsigned kMaxRound = CR_MAX_ROUND;
+constexpr uint8_t kUnmarked = static_cast<uint8_t>(kMaxRound << 1);
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(CR_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+// Graph image copied into LDS by every block that ranks or unranks tuples:
+//   rank_tab u32[K*n]   rank_tab[i*n + x] = C(M-1-(x+i), K-i), M = n+K-1
+//   binom    u32[(K+1)*(M+1)]  binom[j*(M+1) + x] = C(x, j), saturated
+//   in_off   u16[n+1]   in-neighbour offsets
+//   in_nbr   u8[E]      in-neighbours (cop predecessors step back along them)
+//   out_off  u16[n+1], out_nbr u8[E]: out-neighbours (forward team moves)
+struct GraphView {
+    const uint8_t* image;
+    uint32_t image_bytes;
+    uint32_t off_binom, off_in_off, off_in_nbr, off_out_off, off_out_nbr;
+    int32_t n, W, M;
+    uint32_t C;
+    uint32_t ctot_m1;  // C - 1
+};
+
+struct Ctl {
+    uint32_t cnt0[2];  // list lengths: configurations with new cop-turn states
+    uint32_t cnt1[2];  // ... with new robber-turn states
+    unsigned long long lvl[128][2];
+    unsigned long long verdict;
+    unsigned long long retro[3];
+};
+
+struct LevelArgs {
+    uint64_t* won0;
+    uint64_t* won1;
+    uint64_t* fr1;
+    uint64_t* nx0;
+    uint8_t* table;
+    const uint32_t* list0_cur;
+    const uint32_t* list1_cur;
+    uint32_t* list0_nxt;
+    uint32_t* list1_nxt;
+    Ctl* ctl;
+    const uint64_t* out_mask;  // [n][4] robber moves
+    int32_t cur;
+    uint32_t depth_new;
+    int32_t filter;
+};
+
+__device__ __forceinline__ uint8_t won_byte(uint32_t d) { return static_cast<uint8_t>(1u | (d << 1)); }
+
+__device__ __forceinline__ void load_image(uint8_t* lds, const uint8_t* src, uint32_t bytes) {
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* d = reinterpret_cast<uint32_t*>(lds);
+    for (uint32_t i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
+    __syncthreads();
+}
+
+// Combinatorial-number-system decode of a lexicographic rank.
+template <int K>
+__device__ __forceinline__ void unrank(uint32_t rank, const GraphView& gv, const uint32_t* binom,
+                                       uint32_t* cops) {
+    uint32_t s = gv.ctot_m1 - rank;
+    int xmax = gv.M - 1;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const uint32_t* row = binom + (K - i) * (gv.M + 1);
+        int lo = 0, hi = xmax;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (row[mid] <= s) lo = mid; else hi = mid - 1;
+        }
+        s -= row[lo];
+        cops[i] = static_cast<uint32_t>(gv.M - 1 - lo - i);
+        xmax = lo - 1;
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void sort_tuple(uint32_t* t) {
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+#pragma unroll
+        for (int j = 0; j + 1 < K - i; ++j) {
+            const uint32_t a = t[j], b = t[j + 1];
+            t[j] = min(a, b);
+            t[j + 1] = max(a, b);
+        }
+}
+
+template <int K>
+__device__ __forceinline__ uint32_t rank_sorted(const uint32_t* t, const uint32_t* rank_tab, int n,
+                                                uint32_t ctot_m1) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) s += rank_tab[i * n + t[i]];
+    return ctot_m1 - s;
+}
+
+// Decodes product index t into one predecessor tuple (sorted) and returns
+// false when the tuple is a duplicate produced by swapping two stacked cops.
+template <int K>
+__device__ __forceinline__ bool product_tuple(uint32_t t, const uint32_t* cops, const uint32_t* deg,
+                                              const uint32_t* base, const uint8_t* nbr,
+                                              uint32_t* tup) {
+    uint32_t idx[K];
+#pragma unroll
+    for (int i = K - 1; i > 0; --i) {
+        const uint32_t q = t / deg[i];
+        idx[i] = t - q * deg[i];
+        t = q;
+    }
+    idx[0] = t;
+    bool keep = true;
+#pragma unroll
+    for (int i = 0; i + 1 < K; ++i)
+        if (cops[i] == cops[i + 1] && idx[i] > idx[i + 1]) keep = false;
+#pragma unroll
+    for (int i = 0; i < K; ++i) tup[i] = nbr[base[i] + idx[i]];
+    sort_tuple<K>(tup);
+    return keep;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// ---------------------------------------------------------------------------
+// init: reset every state, seed captures (k_cops_visibility.cpp:85-127)
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    load_image(lds, gv.image, gv.image_bytes);
+    const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
+    const int lane = threadIdx.x & 63;
+    const int W = gv.W;
+    const uint32_t nbatch = (gv.C + 63) / 64;
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    unsigned long long caps = 0;
+    for (uint32_t b = gw; b < nbatch; b += gridDim.x * kWavesPerBlock) {
+        const uint32_t row = b * 64 + lane;
+        uint64_t cm[4] = {0, 0, 0, 0};
+        if (row < gv.C) {
+            uint32_t cops[K];
+            unrank<K>(row, gv, binom, cops);
+#pragma unroll
+            for (int i = 0; i < K; ++i) cm[cops[i] >> 6] |= 1ull << (cops[i] & 63);
+            for (int w = 0; w < W; ++w) {
+                const uint64_t o = static_cast<uint64_t>(row) * W + w;
+                a.won0[o] = cm[w];
+                a.won1[o] = cm[w];
+                a.fr1[o] = cm[w];
+                a.nx0[o] = 0;
+            }
+            a.list0_nxt[row] = row;  // level 0: every configuration holds captures
+            a.list1_nxt[row] = row;
+            caps += __popcll(cm[0]) + __popcll(cm[1]) + __popcll(cm[2]) + __popcll(cm[3]);
+        }
+        const uint32_t rows_here = min(64u, gv.C - b * 64);
+        for (uint32_t i = 0; i < rows_here; ++i) {
+            const uint32_t rr = b * 64 + i;
+            uint64_t m[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) m[w] = __shfl(cm[w], static_cast<int>(i));
+            uint16_t* trow = reinterpret_cast<uint16_t*>(a.table + static_cast<uint64_t>(rr) * gv.n * 2);
+            for (int r = lane; r < gv.n; r += 64) {
+                const bool cap = (m[r >> 6] >> (r & 63)) & 1;
+                trow[r] = cap ? static_cast<uint16_t>(0x0101) : static_cast<uint16_t>(kUnmarked * 0x0101);
+            }
+        }
+    }
+    // wave-reduce the capture count
+    for (int off = 32; off > 0; off >>= 1) caps += __shfl_xor(caps, off);
+    if (lane == 0 && caps) {
+        atomicAdd(&a.ctl->lvl[0][0], caps);
+        atomicAdd(&a.ctl->lvl[0][1], caps);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// cop turn, push: frontier robber-turn rows -> predecessors' nx0
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    load_image(lds, gv.image, gv.image_bytes);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctl->cnt0[a.cur ^ 1] = 0;
+        a.ctl->cnt1[a.cur ^ 1] = 0;
+    }
+    const uint32_t* rank_tab = reinterpret_cast<const uint32_t*>(lds);
+    const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
+    const uint16_t* in_off = reinterpret_cast<const uint16_t*>(lds + gv.off_in_off);
+    const uint8_t* in_nbr = lds + gv.off_in_nbr;
+    const int lane = threadIdx.x & 63;
+    const int W = gv.W;
+    const uint32_t count = a.ctl->cnt1[a.cur];
+    for (uint32_t it = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); it < count;
+         it += gridDim.x * kWavesPerBlock) {
+        const uint32_t row = a.list1_cur[it];
+        uint64_t m[4] = {0, 0, 0, 0};
+        for (int w = 0; w < W; ++w) m[w] = a.fr1[static_cast<uint64_t>(row) * W + w];
+        uint32_t cops[K], deg[K], base[K];
+        unrank<K>(row, gv, binom, cops);
+        uint32_t P = 1;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            base[i] = in_off[cops[i]];
+            deg[i] = in_off[cops[i] + 1] - base[i];
+            P *= deg[i];
+        }
+        for (uint32_t t = lane; t < P; t += 64) {
+            uint32_t tup[K];
+            if (!product_tuple<K>(t, cops, deg, base, in_nbr, tup)) continue;
+            const uint64_t pr = rank_sorted<K>(tup, rank_tab, gv.n, gv.ctot_m1);
+            for (int w = 0; w < W; ++w) {
+                uint64_t mw = m[w];
+                if (mw == 0) continue;
+                if (a.filter) {
+                    mw &= ~a.won0[pr * W + w];
+                    if (mw == 0) continue;
+                }
+                atomicOr(reinterpret_cast<unsigned long long*>(a.nx0 + pr * W + w),
+                         static_cast<unsigned long long>(mw));
+            }
+        }
+        if (lane < W) a.fr1[static_cast<uint64_t>(row) * W + lane] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// robber turn: rows with new cop-turn states; r won iff out_mask[r] within won0
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs a) {
+    const int lane = threadIdx.x & 63;
+    uint64_t om[W][W];
+    bool valid[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const int r = lane + 64 * j;
+        valid[j] = r < gv.n;
+#pragma unroll
+        for (int q = 0; q < W; ++q) om[j][q] = valid[j] ? a.out_mask[r * 4 + q] : 0ull;
+    }
+    const uint32_t count = a.ctl->cnt0[a.cur];
+    const uint32_t nbatch = (count + 63) / 64;
+    const uint8_t nb = won_byte(a.depth_new);
+    unsigned long long found = 0;
+    for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
+         b += gridDim.x * kWavesPerBlock) {
+        const uint32_t mine = b * 64 + lane < count ? a.list0_cur[b * 64 + lane] : 0u;
+        const uint32_t rows_here = min(64u, count - b * 64);
+        bool append = false;
+        for (uint32_t i = 0; i < rows_here; ++i) {
+            const uint32_t row = __shfl(mine, static_cast<int>(i));
+            const uint64_t o = static_cast<uint64_t>(row) * W;
+            uint64_t w0[W], w1[W], nw[W];
+#pragma unroll
+            for (int q = 0; q < W; ++q) { w0[q] = a.won0[o + q]; w1[q] = a.won1[o + q]; }
+            uint32_t pop = 0;
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                bool ok = valid[j] && !((w1[j] >> lane) & 1);
+#pragma unroll
+                for (int q = 0; q < W; ++q) ok = ok && ((om[j][q] & ~w0[q]) == 0);
+                nw[j] = __ballot(ok);
+                if (ok) a.table[(static_cast<uint64_t>(row) * gv.n + lane + 64 * j) * 2 + 1] = nb;
+                pop += __popcll(nw[j]);
+            }
+            if (pop) {
+#pragma unroll
+                for (int q = 0; q < W; ++q)
+                    if (lane == q) {
+                        a.won1[o + q] = w1[q] | nw[q];
+                        a.fr1[o + q] = nw[q];
+                    }
+                found += pop;
+                if (lane == static_cast<int>(i)) append = true;
+            }
+        }
+        const uint64_t bal = __ballot(append);
+        if (bal) {
+            uint32_t pos = 0;
+            if (lane == 0) pos = atomicAdd(&a.ctl->cnt1[a.cur ^ 1], static_cast<uint32_t>(__popcll(bal)));
+            pos = __shfl(pos, 0);
+            if (append) a.list1_nxt[pos + __popcll(bal & lanemask_lt())] = mine;
+        }
+    }
+    if (lane == 0 && found) atomicAdd(&a.ctl->lvl[a.depth_new][1], found);
+}
+
+// ---------------------------------------------------------------------------
+// cop turn, finalise: nx0 & ~won0 -> new cop-turn states (full scan of nx0)
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t nbatch = (gv.C + 63) / 64;
+    const uint8_t nb = won_byte(a.depth_new);
+    unsigned long long found = 0;
+    for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
+         b += gridDim.x * kWavesPerBlock) {
+        const uint32_t row = b * 64 + lane;
+        uint64_t x[W];
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            x[q] = row < gv.C ? a.nx0[static_cast<uint64_t>(row) * W + q] : 0ull;
+            any |= x[q] != 0;
+        }
+        uint64_t bal = __ballot(any);
+        bool append = false;
+        while (bal) {
+            const int src = __builtin_ctzll(bal);
+            bal &= bal - 1;
+            const uint32_t rr = b * 64 + src;
+            const uint64_t o = static_cast<uint64_t>(rr) * W;
+            uint64_t nw[W];
+            uint32_t pop = 0;
+#pragma unroll
+            for (int q = 0; q < W; ++q) {
+                const uint64_t xv = __shfl(x[q], src);
+                const uint64_t wv = a.won0[o + q];
+                nw[q] = xv & ~wv;
+                pop += __popcll(nw[q]);
+                if (lane == q) {
+                    a.nx0[o + q] = 0;
+                    if (nw[q]) a.won0[o + q] = wv | nw[q];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                if ((nw[j] >> lane) & 1) a.table[(static_cast<uint64_t>(rr) * gv.n + lane + 64 * j) * 2] = nb;
+            if (pop) {
+                found += pop;
+                if (lane == src) append = true;
+            }
+        }
+        const uint64_t ab = __ballot(append);
+        if (ab) {
+            uint32_t pos = 0;
+            if (lane == 0) pos = atomicAdd(&a.ctl->cnt0[a.cur ^ 1], static_cast<uint32_t>(__popcll(ab)));
+            pos = __shfl(pos, 0);
+            if (append) a.list0_nxt[pos + __popcll(ab & lanemask_lt())] = row;
+        }
+    }
+    if (lane == 0 && found) atomicAdd(&a.ctl->lvl[a.depth_new][0], found);
+}
+
+// ---------------------------------------------------------------------------
+// verdict (k_cops_visibility.cpp:301-354): universal rows -> min worst depth,
+// lowest cId on ties; packed (worst << 32 | cId) atomicMin
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_verdict(GraphView gv, LevelArgs a, int32_t columns) {
+    const int lane = threadIdx.x & 63;
+    uint64_t full[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        const int bits = min(64, gv.n - 64 * q);
+        full[q] = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+    }
+    const uint32_t nbatch = (gv.C + 63) / 64;
+    for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
+         b += gridDim.x * kWavesPerBlock) {
+        const uint32_t row = b * 64 + lane;
+        bool uni = row < gv.C;
+#pragma unroll
+        for (int q = 0; q < W; ++q) uni = uni && a.won0[static_cast<uint64_t>(row) * W + q] == full[q];
+        uint64_t bal = __ballot(uni);
+        while (bal) {
+            const int src = __builtin_ctzll(bal);
+            bal &= bal - 1;
+            const uint32_t rr = b * 64 + src;
+            uint32_t worst = 0;
+            for (int r = lane; r < gv.n; r += 64)
+                worst = max(worst, static_cast<uint32_t>(a.table[(static_cast<uint64_t>(rr) * gv.n + r) * columns] >> 1));
+            for (int off = 32; off > 0; off >>= 1) worst = max(worst, static_cast<uint32_t>(__shfl_xor(worst, off)));
+            if (lane == 0) atomicMin(&a.ctl->verdict, (static_cast<unsigned long long>(worst) << 32) | rr);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// retro-edge count (metric only): |pred(C)| by an LDS hash set per wave
+// ---------------------------------------------------------------------------
+// mode 0: retro-edge count of a solved table over predecessor sets (in-lists);
+// mode 1: total forward team moves sum |T(C)| (out-lists), the reference's
+//         transitions.size() (AuxGraph.h:258-274).
+template <int K>
+__global__ __launch_bounds__(kWave) void k_count_retro(GraphView gv, LevelArgs a, uint32_t slots,
+                                                       uint32_t shift, int32_t mode) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    load_image(lds, gv.image, gv.image_bytes);
+    // slot = (row + 1) << 32 | rank: a slot tagged with another row counts as empty
+    unsigned long long* hs = reinterpret_cast<unsigned long long*>(lds + ((gv.image_bytes + 15) & ~15u));
+    const uint32_t* rank_tab = reinterpret_cast<const uint32_t*>(lds);
+    const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
+    const uint16_t* in_off = reinterpret_cast<const uint16_t*>(lds + gv.off_in_off);
+    const uint16_t* mv_off = reinterpret_cast<const uint16_t*>(lds + (mode ? gv.off_out_off : gv.off_in_off));
+    const uint8_t* mv_nbr = lds + (mode ? gv.off_out_nbr : gv.off_in_nbr);
+    const int lane = threadIdx.x;
+    for (uint32_t i = lane; i < slots; i += 64) hs[i] = 0;
+    __syncthreads();
+    unsigned long long e_cop = 0, e_rob = 0, won = 0;
+    for (uint32_t row = blockIdx.x; row < gv.C; row += gridDim.x) {
+        uint32_t cops[K], deg[K], base[K];
+        unrank<K>(row, gv, binom, cops);
+        uint32_t P = 1;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            base[i] = mv_off[cops[i]];
+            deg[i] = mv_off[cops[i] + 1] - base[i];
+            P *= deg[i];
+        }
+        const unsigned long long tag = static_cast<unsigned long long>(row + 1) << 32;
+        uint32_t distinct = 0;
+        for (uint32_t t = lane; t < P; t += 64) {
+            uint32_t tup[K];
+            if (!product_tuple<K>(t, cops, deg, base, mv_nbr, tup)) continue;
+            const uint32_t pr = rank_sorted<K>(tup, rank_tab, gv.n, gv.ctot_m1);
+            const unsigned long long key = tag | pr;
+            uint32_t h = ((pr + 1u) * 2654435761u) >> shift;
+            for (;;) {
+                unsigned long long cur = hs[h];
+                if (cur == key) break;
+                if ((cur >> 32) != (tag >> 32)) {
+                    const unsigned long long seen = atomicCAS(&hs[h], cur, key);
+                    if (seen == cur) { ++distinct; break; }
+                    continue;  // lost the race: re-examine this slot
+                }
+                h = (h + 1) & (slots - 1);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) distinct += __shfl_xor(distinct, off);
+        if (mode) {
+            e_cop += distinct;
+            continue;
+        }
+        uint32_t rob = 0, cop_e = 0, w = 0;
+        for (int r = lane; r < gv.n; r += 64) {
+            const uint8_t b0 = a.table[(static_cast<uint64_t>(row) * gv.n + r) * 2];
+            const uint8_t b1 = a.table[(static_cast<uint64_t>(row) * gv.n + r) * 2 + 1];
+            if (b0 & 1) { ++w; if ((b0 >> 1) <= kMaxRound - 2) cop_e += in_off[r + 1] - in_off[r]; }
+            if (b1 & 1) { ++w; if ((b1 >> 1) <= kMaxRound - 2) ++rob; }
+        }
+        for (int off = 32; off > 0; off >>= 1) rob += __shfl_xor(rob, off);
+        e_cop += static_cast<unsigned long long>(rob) * distinct;
+        e_rob += cop_e;
+        won += w;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        e_rob += __shfl_xor(e_rob, off);
+        won += __shfl_xor(won, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&a.ctl->retro[0], e_cop);
+        atomicAdd(&a.ctl->retro[1], e_rob);
+        atomicAdd(&a.ctl->retro[2], won);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct KTime {
+    double ms = 0;
+    int64_t launches = 0;
+};
+
+enum Stage { kEmpty = 0, kLoaded, kBuilt, kInitialized, kSolved };
+
+template <typename T>
+T* dev_alloc(uint64_t count, uint64_t& total, int& rc) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    const hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+        rc = fail(CR_ERR_DEVICE, "hipMalloc(%llu bytes): %s",
+                  static_cast<unsigned long long>(count * sizeof(T)), hipGetErrorString(e));
+        return nullptr;
+    }
+    total += count * sizeof(T);
+    return static_cast<T*>(p);
+}
+
+}  // namespace
+
+struct cr_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Stage stage = kEmpty;
+    cr::Adjacency g;
+    std::string path;
+    int32_t k = 0, p = 1, variant = CR_VARIANT_COLUMNS;
+    uint64_t C = 0, S = 0;
+    int32_t W = 0, columns = 2;
+    // device memory
+    uint8_t* d_table = nullptr;
+    uint64_t* d_won0 = nullptr;
+    uint64_t* d_won1 = nullptr;
+    uint64_t* d_fr1 = nullptr;
+    uint64_t* d_nx0 = nullptr;
+    uint32_t* d_list[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [turn][buffer]
+    Ctl* d_ctl = nullptr;
+    uint8_t* d_image = nullptr;
+    uint64_t* d_outmask = nullptr;
+    Ctl* h_ctl = nullptr;  // pinned
+    uint64_t device_bytes = 0;
+    GraphView gv{};
+    int32_t filter = 1;
+    // results
+    std::vector<uint64_t> pass_counts;
+    cr_stats stats{};
+    bool timing = false;
+    std::map<std::string, KTime> ktimes;
+    std::vector<std::string> korder;
+
+    ~cr_solver() { release(); }
+
+    void release() {
+        hipFree(d_table);
+        hipFree(d_won0);
+        hipFree(d_won1);
+        hipFree(d_fr1);
+        hipFree(d_nx0);
+        for (auto& t : d_list)
+            for (auto& b : t) { hipFree(b); b = nullptr; }
+        hipFree(d_ctl);
+        hipFree(d_image);
+        hipFree(d_outmask);
+        d_table = nullptr;
+        d_won0 = d_won1 = d_fr1 = d_nx0 = nullptr;
+        d_ctl = nullptr;
+        d_image = nullptr;
+        d_outmask = nullptr;
+        device_bytes = 0;
+    }
+};
+
+namespace {
+
+int check_solver(const cr_solver* s) {
+    if (s == nullptr) return fail(CR_ERR_ARG, "null solver");
+    return 0;
+}
+
+// Dispatch helpers over the compile-time cop count K and row width W.
+template <template <int> class F, typename... Args>
+int dispatch_k(int k, Args&&... args) {
+    switch (k) {
+        case 1: return F<1>::run(args...);
+        case 2: return F<2>::run(args...);
+        case 3: return F<3>::run(args...);
+        case 4: return F<4>::run(args...);
+        case 5: return F<5>::run(args...);
+        case 6: return F<6>::run(args...);
+        case 7: return F<7>::run(args...);
+        case 8: return F<8>::run(args...);
+        default: return fail(CR_ERR_K_RANGE, "k=%d outside 1..%d", k, CR_MAX_K);
+    }
+}
+
+template <template <int> class F, typename... Args>
+int dispatch_w(int w, Args&&... args) {
+    switch (w) {
+        case 1: return F<1>::run(args...);
+        case 2: return F<2>::run(args...);
+        case 3: return F<3>::run(args...);
+        case 4: return F<4>::run(args...);
+        default: return fail(CR_ERR_ARG, "row width %d", w);
+    }
+}
+
+// Blocks for `waves` wavefronts of work, capped (kernels loop grid-stride).
+// Kernels that copy the graph image into LDS use a smaller cap so the copy is
+// amortised over many rows.
+uint32_t grid_for(uint64_t waves, uint64_t cap = 8192) {
+    const uint64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(blocks, cap)));
+}
+
+struct Timer {
+    cr_solver* s;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer(cr_solver* s_, const char* n) : s(s_), name(n) {
+        if (s->timing) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, s->stream);
+        }
+    }
+    void stop() {
+        if (!s->timing) return;
+        hipEventRecord(b, s->stream);
+    }
+    // called after the stream has been synchronised
+    void collect() {
+        if (!s->timing || a == nullptr) return;
+        float ms = 0;
+        hipEventElapsedTime(&ms, a, b);
+        auto it = s->ktimes.find(name);
+        if (it == s->ktimes.end()) {
+            s->korder.push_back(name);
+            it = s->ktimes.emplace(name, KTime{}).first;
+        }
+        it->second.ms += ms;
+        it->second.launches += 1;
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        a = b = nullptr;
+    }
+};
+
+template <int K>
+struct LaunchInit {
+    static int run(cr_solver* s, LevelArgs a) {
+        hipLaunchKernelGGL(k_init<K>, dim3(grid_for((s->C + 63) / 64, 2048)), dim3(kBlock), s->gv.image_bytes,
+                           s->stream, s->gv, a);
+        return 0;
+    }
+};
+
+template <int K>
+struct LaunchPush {
+    static int run(cr_solver* s, LevelArgs a, uint32_t rows) {
+        hipLaunchKernelGGL(k_cop_push<K>, dim3(grid_for(rows, 2048)), dim3(kBlock), s->gv.image_bytes, s->stream,
+                           s->gv, a);
+        return 0;
+    }
+};
+
+template <int W>
+struct LaunchRobber {
+    static int run(cr_solver* s, LevelArgs a, uint32_t rows) {
+        hipLaunchKernelGGL(k_robber_step<W>, dim3(grid_for((rows + 63) / 64)), dim3(kBlock), 0, s->stream,
+                           s->gv, a);
+        return 0;
+    }
+};
+
+template <int W>
+struct LaunchFinalize {
+    static int run(cr_solver* s, LevelArgs a) {
+        hipLaunchKernelGGL(k_cop_finalize<W>, dim3(grid_for((s->C + 63) / 64)), dim3(kBlock), 0, s->stream,
+                           s->gv, a);
+        return 0;
+    }
+};
+
+template <int W>
+struct LaunchVerdict {
+    static int run(cr_solver* s, LevelArgs a) {
+        hipLaunchKernelGGL(k_verdict<W>, dim3(grid_for((s->C + 63) / 64)), dim3(kBlock), 0, s->stream, s->gv,
+                           a, s->columns);
+        return 0;
+    }
+};
+
+template <int K>
+struct LaunchRetro {
+    static int run(cr_solver* s, LevelArgs a, uint32_t slots, int32_t mode) {
+        const uint32_t lds = ((s->gv.image_bytes + 15) & ~15u) + slots * 8;
+        uint32_t shift = 32;
+        for (uint32_t v = slots; v > 1; v >>= 1) --shift;
+        hipFuncSetAttribute(reinterpret_cast<const void*>(k_count_retro<K>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(s->C, 8192));
+        hipLaunchKernelGGL(k_count_retro<K>, dim3(blocks), dim3(kWave), lds, s->stream, s->gv, a, slots, shift, mode);
+        return 0;
+    }
+};
+
+LevelArgs level_args(cr_solver* s, int cur, uint32_t depth_new) {
+    LevelArgs a{};
+    a.won0 = s->d_won0;
+    a.won1 = s->d_won1;
+    a.fr1 = s->d_fr1;
+    a.nx0 = s->d_nx0;
+    a.table = s->d_table;
+    a.list0_cur = s->d_list[0][cur];
+    a.list1_cur = s->d_list[1][cur];
+    a.list0_nxt = s->d_list[0][cur ^ 1];
+    a.list1_nxt = s->d_list[1][cur ^ 1];
+    a.ctl = s->d_ctl;
+    a.out_mask = s->d_outmask;
+    a.cur = cur;
+    a.depth_new = depth_new;
+    a.filter = s->filter;
+    return a;
+}
+
+// Builds the LDS image and uploads the graph tables.
+int upload_graph(cr_solver* s) {
+    const int n = s->g.n, K = s->k, M = n + K - 1;
+    std::vector<uint8_t> img;
+    auto align4 = [&] { while (img.size() % 4) img.push_back(0); };
+    auto put32 = [&](uint32_t v) { const uint8_t* b = reinterpret_cast<const uint8_t*>(&v); img.insert(img.end(), b, b + 4); };
+    for (int i = 0; i < K; ++i)
+        for (int x = 0; x < n; ++x)
+            put32(static_cast<uint32_t>(std::min<uint64_t>(cr::binom(M - 1 - (x + i), K - i), 0xffffffffu)));
+    const uint32_t off_binom = static_cast<uint32_t>(img.size());
+    for (int j = 0; j <= K; ++j)
+        for (int x = 0; x <= M; ++x)
+            put32(static_cast<uint32_t>(std::min<uint64_t>(cr::binom(x, j), 0xffffffffu)));
+    const uint32_t off_in_off = static_cast<uint32_t>(img.size());
+    uint32_t acc = 0;
+    for (int v = 0; v <= n; ++v) {
+        const uint16_t o = static_cast<uint16_t>(acc);
+        img.push_back(static_cast<uint8_t>(o & 0xff));
+        img.push_back(static_cast<uint8_t>(o >> 8));
+        if (v < n) acc += static_cast<uint32_t>(s->g.in[v].size());
+    }
+    align4();
+    const uint32_t off_in_nbr = static_cast<uint32_t>(img.size());
+    for (int v = 0; v < n; ++v) img.insert(img.end(), s->g.in[v].begin(), s->g.in[v].end());
+    align4();
+    const uint32_t off_out_off = static_cast<uint32_t>(img.size());
+    acc = 0;
+    for (int v = 0; v <= n; ++v) {
+        const uint16_t o = static_cast<uint16_t>(acc);
+        img.push_back(static_cast<uint8_t>(o & 0xff));
+        img.push_back(static_cast<uint8_t>(o >> 8));
+        if (v < n) acc += static_cast<uint32_t>(s->g.out[v].size());
+    }
+    align4();
+    const uint32_t off_out_nbr = static_cast<uint32_t>(img.size());
+    for (int v = 0; v < n; ++v) img.insert(img.end(), s->g.out[v].begin(), s->g.out[v].end());
+    align4();
+    if (img.size() > 150 * 1024) return fail(CR_ERR_UNSUPPORTED, "graph image of %zu bytes exceeds LDS", img.size());
+    int rc = 0;
+    s->d_image = dev_alloc<uint8_t>(img.size(), s->device_bytes, rc);
+    if (!s->d_image) return rc;
+    HIP_TRY(hipMemcpy(s->d_image, img.data(), img.size(), hipMemcpyHostToDevice));
+    std::vector<uint64_t> om(static_cast<size_t>(n) * 4, 0);
+    for (int u = 0; u < n; ++u)
+        for (uint8_t v : s->g.out[u]) om[u * 4 + (v >> 6)] |= 1ull << (v & 63);
+    s->d_outmask = dev_alloc<uint64_t>(om.size(), s->device_bytes, rc);
+    if (!s->d_outmask) return rc;
+    HIP_TRY(hipMemcpy(s->d_outmask, om.data(), om.size() * 8, hipMemcpyHostToDevice));
+    s->gv.image = s->d_image;
+    s->gv.image_bytes = static_cast<uint32_t>(img.size());
+    s->gv.off_binom = off_binom;
+    s->gv.off_in_off = off_in_off;
+    s->gv.off_in_nbr = off_in_nbr;
+    s->gv.off_out_off = off_out_off;
+    s->gv.off_out_nbr = off_out_nbr;
+    s->gv.n = n;
+    s->gv.W = s->W;
+    s->gv.M = M;
+    s->gv.C = static_cast<uint32_t>(s->C);
+    s->gv.ctot_m1 = static_cast<uint32_t>(s->C - 1);
+    return 0;
+}
+
+int set_lds_limits(cr_solver* s) {
+    const int bytes = static_cast<int>(s->gv.image_bytes);
+    if (bytes <= 64 * 1024) return 0;
+    const void* fns[] = {
+        reinterpret_cast<const void*>(k_init<1>), reinterpret_cast<const void*>(k_init<2>),
+        reinterpret_cast<const void*>(k_init<3>), reinterpret_cast<const void*>(k_init<4>),
+        reinterpret_cast<const void*>(k_init<5>), reinterpret_cast<const void*>(k_init<6>),
+        reinterpret_cast<const void*>(k_init<7>), reinterpret_cast<const void*>(k_init<8>),
+        reinterpret_cast<const void*>(k_cop_push<1>), reinterpret_cast<const void*>(k_cop_push<2>),
+        reinterpret_cast<const void*>(k_cop_push<3>), reinterpret_cast<const void*>(k_cop_push<4>),
+        reinterpret_cast<const void*>(k_cop_push<5>), reinterpret_cast<const void*>(k_cop_push<6>),
+        reinterpret_cast<const void*>(k_cop_push<7>), reinterpret_cast<const void*>(k_cop_push<8>)};
+    for (const void* f : fns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    return 0;
+}
+
+int do_load(cr_solver* s, int32_t n, const uint8_t* adj, int32_t k, int32_t p) {
+    s->release();
+    s->stage = kEmpty;
+    if (n <= 0) return fail(CR_ERR_EMPTY_GRAPH, "Graph is empty or failed to load.");
+    if (n > CR_MAX_N) return fail(CR_ERR_GRAPH_LIMIT, "graph has %d vertices; the format limit is %d", n, CR_MAX_N);
+    if (k <= 0 || k > CR_MAX_K) return fail(CR_ERR_K_RANGE, "k=%d outside 1..%d", k, CR_MAX_K);
+    if (p < 1) return fail(CR_ERR_P_RANGE, "p=%d must be >= 1", p);
+    s->g.build(n, adj);
+    for (int v = 0; v < n; ++v)
+        if (s->g.out[v].empty())
+            return fail(CR_ERR_ISOLATED_VERTEX, "vertex %d has no move; the reference's move generation is undefined there", v);
+    s->k = k;
+    s->p = p;
+    s->stage = kLoaded;
+    return 0;
+}
+
+int do_build(cr_solver* s) {
+    if (s->stage < kLoaded) return fail(CR_ERR_ORDER, "buildAuxGraph before loadGraphFile");
+    s->release();
+    const int n = s->g.n;
+    s->C = cr::config_count(n, s->k);
+    if (s->C == 0) return fail(CR_ERR_K_RANGE, "no configuration for k=%d", s->k);
+    if (s->C >= 0xffffffffull) return fail(CR_ERR_TOO_LARGE, "%llu configurations exceed 32-bit ids", static_cast<unsigned long long>(s->C));
+    if (s->variant != CR_VARIANT_COLUMNS || s->p != 1)
+        return fail(CR_ERR_UNSUPPORTED, "device solver supports the plain game (column solver, p = 1) only");
+    uint64_t prod = 1;
+    for (int i = 0; i < s->k; ++i) prod *= static_cast<uint64_t>(std::max(1, s->g.max_in));
+    if (prod >= (1ull << 32)) return fail(CR_ERR_TOO_LARGE, "team-move product %llu too large", static_cast<unsigned long long>(prod));
+    s->columns = 2 * s->p;
+    s->S = s->C * static_cast<uint64_t>(n) * s->columns;
+    s->W = (n + 63) / 64;
+    const uint64_t plane = s->C * s->W;
+    const uint64_t need = s->S + 4 * plane * 8 + 4 * s->C * 4 + (1 << 20);
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    if (need > freeb) return fail(CR_ERR_TOO_LARGE, "state space needs %llu bytes; %zu free", static_cast<unsigned long long>(need), freeb);
+    int rc = 0;
+    if (!(s->d_table = dev_alloc<uint8_t>(s->S, s->device_bytes, rc))) return rc;
+    if (!(s->d_won0 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
+    if (!(s->d_won1 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
+    if (!(s->d_fr1 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
+    if (!(s->d_nx0 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
+    for (auto& t : s->d_list)
+        for (auto& b : t)
+            if (!(b = dev_alloc<uint32_t>(s->C, s->device_bytes, rc))) return rc;
+    if (!(s->d_ctl = dev_alloc<Ctl>(1, s->device_bytes, rc))) return rc;
+    if (!s->h_ctl) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_ctl), sizeof(Ctl), hipHostMallocDefault));
+    if ((rc = upload_graph(s)) != 0) return rc;
+    if ((rc = set_lds_limits(s)) != 0) return rc;
+    s->stage = kBuilt;
+    return 0;
+}
+
+// zeroes the control block with a kernel-free host copy (never captured)
+int reset_ctl(cr_solver* s) {
+    std::memset(s->h_ctl, 0, sizeof(Ctl));
+    s->h_ctl->verdict = ~0ull;
+    s->h_ctl->cnt0[0] = static_cast<uint32_t>(s->C);  // level 0: every row holds captures
+    s->h_ctl->cnt1[0] = static_cast<uint32_t>(s->C);
+    HIP_TRY(hipMemcpyAsync(s->d_ctl, s->h_ctl, sizeof(Ctl), hipMemcpyHostToDevice, s->stream));
+    return 0;
+}
+
+int do_init(cr_solver* s, uint64_t* base) {
+    if (s->stage < kBuilt) return fail(CR_ERR_ORDER, "initializeCaptures before buildAuxGraph");
+    int rc = reset_ctl(s);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    // level 0 lists are written into buffer 0 (the "next" of cur = 1)
+    LevelArgs a = level_args(s, 1, 0);
+    Timer t(s, "init");
+    if ((rc = dispatch_k<LaunchInit>(s->k, s, a)) != 0) return rc;
+    t.stop();
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    t.collect();
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    s->stats = cr_stats{};
+    s->stats.init_ms = ms;
+    s->stats.base_captures = s->h_ctl->lvl[0][0];
+    if (base) *base = s->stats.base_captures;
+    s->stage = kInitialized;
+    return 0;
+}
+
+void passes_from_levels(cr_solver* s) {
+    const auto& lv = s->h_ctl->lvl;
+    s->pass_counts.clear();
+    for (int i = 1;; ++i) {
+        uint64_t c = 0;
+        const int c0 = 2 * i - 2, c1 = 2 * i - 1, r0 = 2 * i - 1, r1 = 2 * i;
+        if (c0 > 0 && c0 < 127) c += lv[c0][0];
+        if (c1 < 127) c += lv[c1][0];
+        if (r0 < 127) c += lv[r0][1];
+        if (r1 < 127) c += lv[r1][1];
+        s->pass_counts.push_back(c);
+        if (c == 0) break;
+    }
+}
+
+int do_loop(cr_solver* s) {
+    if (s->stage < kInitialized) return fail(CR_ERR_ORDER, "mainLoop before initializeCaptures");
+    s->ktimes.clear();
+    s->korder.clear();
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    uint32_t n0 = static_cast<uint32_t>(s->C), n1 = static_cast<uint32_t>(s->C);
+    int levels = 0;
+    int rc = 0;
+    for (uint32_t d = 0; d + 1 < kMaxRound; ++d) {
+        const int c = static_cast<int>(d & 1);  // level d reads list buffer d & 1
+        LevelArgs a = level_args(s, c, d + 1);
+        Timer t1(s, "cop_push");  // launched even with no rows: it resets the next counters
+        if ((rc = dispatch_k<LaunchPush>(s->k, s, a, n1)) != 0) return rc;
+        t1.stop();
+        Timer t2(s, "robber_step");
+        if (n0 && (rc = dispatch_w<LaunchRobber>(s->W, s, a, n0)) != 0) return rc;
+        t2.stop();
+        Timer t3(s, "cop_finalize");
+        if ((rc = dispatch_w<LaunchFinalize>(s->W, s, a)) != 0) return rc;
+        t3.stop();
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl, offsetof(Ctl, lvl), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        t1.collect();
+        t2.collect();
+        t3.collect();
+        ++levels;
+        n0 = s->h_ctl->cnt0[c ^ 1];
+        n1 = s->h_ctl->cnt1[c ^ 1];
+        if (n0 == 0 && n1 == 0) break;
+    }
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    s->stats.loop_ms = ms;
+    s->stats.n_levels = levels;
+    uint64_t won = 0;
+    for (int d = 0; d < 127; ++d) won += s->h_ctl->lvl[d][0] + s->h_ctl->lvl[d][1];
+    s->stats.won_states = won;
+    passes_from_levels(s);
+    s->stats.n_passes = static_cast<int32_t>(s->pass_counts.size());
+    s->stage = kSolved;
+    return 0;
+}
+
+int count_moves(cr_solver* s, int32_t mode, unsigned long long* r) {
+    HIP_TRY(hipSetDevice(s->device));
+    uint64_t prod = 1;
+    const int deg = std::max(1, mode ? s->g.max_out : s->g.max_in);
+    for (int i = 0; i < s->k; ++i) prod *= static_cast<uint64_t>(deg);
+    uint32_t slots = 64;
+    while (slots < 2 * prod) slots <<= 1;
+    const uint64_t lds = ((s->gv.image_bytes + 15) & ~15u) + static_cast<uint64_t>(slots) * 8;
+    if (lds > 160 * 1024)
+        return fail(CR_ERR_UNSUPPORTED, "move count needs %llu bytes of LDS", static_cast<unsigned long long>(lds));
+    unsigned long long zero[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(s->d_ctl->retro, zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
+    LevelArgs a = level_args(s, 0, 0);
+    if (int rc = dispatch_k<LaunchRetro>(s->k, s, a, slots, mode)) return rc;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(r, s->d_ctl->retro, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* cr_last_error(void) { return g_err; }
+
+int cr_solver_create(int device, cr_solver** out) {
+    if (out == nullptr) return fail(CR_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CR_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count) return fail(CR_ERR_ARG, "device %d of %d", device, count);
+    HIP_TRY(hipSetDevice(device));
+    auto s = std::make_unique<cr_solver>();
+    s->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    *out = s.release();
+    g_err[0] = '\0';
+    return CR_OK;
+}
+
+void cr_solver_destroy(cr_solver* s) {
+    if (!s) return;
+    hipSetDevice(s->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    s->release();
+    if (s->h_ctl) hipHostFree(s->h_ctl);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int32_t cr_parse_matrix(const uint8_t* text, uint64_t len, uint8_t* adj, int32_t cap) {
+    return cr::parse_matrix(text, len, adj, cap);
+}
+
+int cr_load_graph_file(cr_solver* s, const char* filename, int32_t k, int32_t p) {
+    if (int rc = check_solver(s)) return rc;
+    if (filename == nullptr) return fail(CR_ERR_ARG, "null filename");
+    std::FILE* f = std::fopen(filename, "rb");
+    if (!f) return fail(CR_ERR_EMPTY_GRAPH, "Graph is empty or failed to load: %s", filename);
+    std::vector<uint8_t> text;
+    uint8_t buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) text.insert(text.end(), buf, buf + got);
+    std::fclose(f);
+    const int32_t n = cr::parse_matrix(text.data(), text.size(), nullptr, 0);
+    if (n <= 0) return fail(CR_ERR_EMPTY_GRAPH, "Graph is empty or failed to load: %s", filename);
+    if (n > CR_MAX_N) return fail(CR_ERR_GRAPH_LIMIT, "graph has %d vertices; the format limit is %d", n, CR_MAX_N);
+    std::vector<uint8_t> adj(static_cast<size_t>(n) * n);
+    cr::parse_matrix(text.data(), text.size(), adj.data(), n);
+    s->path = filename;
+    return do_load(s, n, adj.data(), k, p);
+}
+
+int cr_load_graph_matrix(cr_solver* s, int32_t n, const uint8_t* adj, int32_t k, int32_t p) {
+    if (int rc = check_solver(s)) return rc;
+    if (adj == nullptr && n > 0) return fail(CR_ERR_ARG, "null matrix");
+    s->path = "matrix";
+    return do_load(s, n, adj, k, p);
+}
+
+int cr_set_variant(cr_solver* s, int32_t variant) {
+    if (int rc = check_solver(s)) return rc;
+    if (variant != CR_VARIANT_COLUMNS && variant != CR_VARIANT_MACRO) return fail(CR_ERR_ARG, "variant %d", variant);
+    s->variant = variant;
+    if (s->stage > kLoaded) s->stage = kLoaded;
+    return 0;
+}
+
+int cr_build_aux_graph(cr_solver* s) {
+    if (int rc = check_solver(s)) return rc;
+    HIP_TRY(hipSetDevice(s->device));
+    return do_build(s);
+}
+
+int cr_initialize_captures(cr_solver* s, uint64_t* base_captures) {
+    if (int rc = check_solver(s)) return rc;
+    HIP_TRY(hipSetDevice(s->device));
+    return do_init(s, base_captures);
+}
+
+int cr_main_loop(cr_solver* s, uint64_t* pass_counts, int32_t max_passes, int32_t* n_passes) {
+    if (int rc = check_solver(s)) return rc;
+    HIP_TRY(hipSetDevice(s->device));
+    if (int rc = do_loop(s)) return rc;
+    if (pass_counts)
+        for (int32_t i = 0; i < max_passes && i < static_cast<int32_t>(s->pass_counts.size()); ++i)
+            pass_counts[i] = s->pass_counts[i];
+    if (n_passes) *n_passes = static_cast<int32_t>(s->pass_counts.size());
+    return 0;
+}
+
+int cr_find_final_result(cr_solver* s, cr_verdict* out) {
+    if (int rc = check_solver(s)) return rc;
+    if (out == nullptr) return fail(CR_ERR_ARG, "null verdict");
+    if (s->stage < kSolved) return fail(CR_ERR_ORDER, "findFinalResult before mainLoop");
+    HIP_TRY(hipSetDevice(s->device));
+    const unsigned long long init = ~0ull;
+    HIP_TRY(hipMemcpyAsync(&s->d_ctl->verdict, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
+    LevelArgs a = level_args(s, 0, 0);
+    if (int rc = dispatch_w<LaunchVerdict>(s->W, s, a)) return rc;
+    HIP_TRY(hipGetLastError());
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, &s->d_ctl->verdict, sizeof v, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::memset(out, 0, sizeof *out);
+    const uint32_t worst = static_cast<uint32_t>(v >> 32);
+    if (v != ~0ull && worst < kMaxRound) {
+        out->win = 1;
+        out->plys = static_cast<int32_t>(worst);
+        out->best_cid = v & 0xffffffffull;
+        cr::unrank_tuple(out->best_cid, s->g.n, s->k, out->positions);
+    } else {
+        out->win = 0;
+        out->plys = -1;
+    }
+    return 0;
+}
+
+int cr_save_aux_graph(cr_solver* s, const char* path) {
+    if (int rc = check_solver(s)) return rc;
+    if (path == nullptr) return fail(CR_ERR_ARG, "null path");
+    if (s->stage < kBuilt) return fail(CR_ERR_ORDER, "outputData before buildAuxGraph");
+    HIP_TRY(hipSetDevice(s->device));
+    auto fetch = [s](uint64_t off, uint64_t cnt, uint8_t* dst) -> int {
+        if (hipMemcpyAsync(dst, s->d_table + off, cnt, hipMemcpyDeviceToHost, s->stream) != hipSuccess) return -1;
+        return hipStreamSynchronize(s->stream) == hipSuccess ? 0 : -1;
+    };
+    const int rc = cr::write_aux_blob(path, s->g, s->k, s->columns, false, false, s->S, fetch);
+    if (rc != 0) return fail(rc, "could not write %s", path);
+    return 0;
+}
+
+int cr_get_info(const cr_solver* s, cr_info* out) {
+    if (int rc = check_solver(s)) return rc;
+    if (out == nullptr) return fail(CR_ERR_ARG, "null info");
+    std::memset(out, 0, sizeof *out);
+    out->n = s->g.n;
+    out->k = s->k;
+    out->p = s->p;
+    out->columns = s->variant == CR_VARIANT_MACRO ? 1 : 2 * s->p;
+    out->variant = s->variant;
+    out->max_degree = s->g.max_out;
+    out->config_count = s->stage >= kLoaded ? cr::config_count(s->g.n, s->k) : 0;
+    out->num_states = out->config_count * static_cast<uint64_t>(s->g.n) * out->columns;
+    out->device_bytes = s->device_bytes;
+    return 0;
+}
+
+int cr_get_stats(const cr_solver* s, cr_stats* out) {
+    if (int rc = check_solver(s)) return rc;
+    if (out == nullptr) return fail(CR_ERR_ARG, "null stats");
+    *out = s->stats;
+    return 0;
+}
+
+int cr_copy_states(cr_solver* s, uint8_t* dst, uint64_t offset, uint64_t count) {
+    if (int rc = check_solver(s)) return rc;
+    if (s->stage < kBuilt) return fail(CR_ERR_ORDER, "no state space");
+    if (dst == nullptr || offset > s->S || count > s->S - offset) return fail(CR_ERR_ARG, "range outside the %llu states", static_cast<unsigned long long>(s->S));
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpyAsync(dst, s->d_table + offset, count, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int cr_copy_configs(cr_solver* s, uint8_t* dst, uint64_t first, uint64_t count) {
+    if (int rc = check_solver(s)) return rc;
+    if (s->stage < kLoaded) return fail(CR_ERR_ORDER, "no graph");
+    const uint64_t C = cr::config_count(s->g.n, s->k);
+    if (dst == nullptr || first > C || count > C - first) return fail(CR_ERR_ARG, "config range");
+    for (uint64_t i = 0; i < count; ++i) cr::unrank_tuple(first + i, s->g.n, s->k, dst + i * s->k);
+    return 0;
+}
+
+int cr_level_counts(cr_solver* s, uint64_t* out) {
+    if (int rc = check_solver(s)) return rc;
+    if (out == nullptr) return fail(CR_ERR_ARG, "null output");
+    if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    for (int d = 0; d < 127; ++d) {
+        out[2 * d] = s->h_ctl->lvl[d][0];
+        out[2 * d + 1] = s->h_ctl->lvl[d][1];
+    }
+    return 0;
+}
+
+int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_t* won) {
+    if (int rc = check_solver(s)) return rc;
+    if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    unsigned long long r[3];
+    if (int rc = count_moves(s, 0, r)) return rc;
+    if (e_cop) *e_cop = r[0];
+    if (e_rob) *e_rob = r[1];
+    if (won) *won = r[2];
+    return 0;
+}
+
+int cr_count_transitions(cr_solver* s, uint64_t* total) {
+    if (int rc = check_solver(s)) return rc;
+    if (total == nullptr) return fail(CR_ERR_ARG, "null output");
+    if (s->stage < kBuilt) return fail(CR_ERR_ORDER, "count before buildAuxGraph");
+    unsigned long long r[3];
+    if (int rc = count_moves(s, 1, r)) return rc;
+    *total = r[0];
+    return 0;
+}
+
+int cr_set_kernel_timing(cr_solver* s, int32_t enabled) {
+    if (int rc = check_solver(s)) return rc;
+    s->timing = enabled != 0;
+    return 0;
+}
+
+int cr_kernel_times(cr_solver* s, char* names_out, int32_t names_cap, double* ms, int64_t* launches,
+                    int32_t cap, int32_t* count) {
+    if (int rc = check_solver(s)) return rc;
+    int32_t i = 0;
+    std::string names;
+    for (const auto& nm : s->korder) {
+        if (i >= cap) break;
+        const KTime& t = s->ktimes[nm];
+        if (ms) ms[i] = t.ms;
+        if (launches) launches[i] = t.launches;
+        names += nm;
+        names.push_back('\0');
+        ++i;
+    }
+    if (names_out && names_cap > 0) {
+        const size_t m = std::min<size_t>(names.size(), static_cast<size_t>(names_cap));
+        std::memcpy(names_out, names.data(), m);
+    }
+    if (count) *count = i;
+    return 0;
+}
+
+int cr_cache_key(const char* graph_path, int32_t k, int32_t p, int32_t variant, char* out, int32_t cap) {
+    if (graph_path == nullptr || out == nullptr || cap <= 0) return fail(CR_ERR_ARG, "bad arguments");
+    const std::string algo = variant == CR_VARIANT_MACRO ? "k_cops_v2_" + std::to_string(p) + "vis"
+                                                         : "k_cops_" + std::to_string(p) + "vis";
+    const std::string key = cr::cache_key(algo, graph_path, k, p, false, false);
+    if (static_cast<int32_t>(key.size()) >= cap) return fail(CR_ERR_ARG, "buffer too small");
+    std::memcpy(out, key.c_str(), key.size() + 1);
+    return static_cast<int>(key.size());
+}
+
+}  // extern "C"

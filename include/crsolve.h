@@ -1,0 +1,162 @@
+/*
+ * crsolve.h -- C ABI of libcrsolve.so, the MI355X (gfx950) k-cop retrograde
+ * solver.
+ *
+ * The reference has no FFI: its solver boundary is six C++ free functions that
+ * app/main.cpp:11-16 declares and each solver translation unit defines
+ * (solvers/k_cops_visibility.cpp, solvers/k_cops_visibility_v2.cpp). Every
+ * entry point below replaces one of those steps, or one accessor that the
+ * steps reach through AuxGraph/CacheManager; the comment on each cites the
+ * reference interface it stands in for. Paths are relative to the reference
+ * repository root.
+ *
+ * Conventions
+ *   - Status: 0 = success, negative CR_ERR_* on failure (the reference's steps
+ *     return bool true on failure, app/main.cpp:29). The message of the last
+ *     failure on the calling thread is available from cr_last_error().
+ *   - Ownership: the caller owns every pointer it passes in. The library owns
+ *     the device memory of a cr_solver until cr_solver_destroy.
+ *   - Threading: one cr_solver is not reentrant (like the reference's TU
+ *     globals, k_cops_visibility.cpp:28-35); distinct solvers may be used
+ *     from distinct threads. Each solver owns one HIP stream.
+ *   - State tables use the reference's layout and byte encoding: index
+ *     (cId*N + r)*columns + col, byte = marked | markedRound << 1
+ *     (DataItem, k_cops_visibility.cpp:20-23 with GCC bit-field order).
+ */
+#ifndef CRSOLVE_H
+#define CRSOLVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CR_MAX_K 8       /* cops per configuration supported on the device */
+#define CR_MAX_N 255     /* vertex ids are uint8 with 255 as terminator (AdjacencyList.h) */
+#define CR_MAX_ROUND 127 /* MAX_ROUND_COUNT, k_cops_visibility.cpp:26 */
+
+enum cr_status {
+    CR_OK = 0,
+    CR_ERR_ARG = -1,             /* null pointer / bad argument */
+    CR_ERR_IO = -2,              /* file unreadable or unwritable */
+    CR_ERR_EMPTY_GRAPH = -3,     /* nodeCount == 0 (k_cops_visibility.cpp:47-50) */
+    CR_ERR_GRAPH_LIMIT = -4,     /* N > 255 (k_cops_visibility_v2.cpp:348-351) */
+    CR_ERR_ISOLATED_VERTEX = -5, /* a vertex with no move (AuxGraph.h:217-221 reads an
+                                    uninitialised option there) */
+    CR_ERR_K_RANGE = -6,         /* k <= 0, k > CR_MAX_K, or no configuration (AuxGraph.h:118-122) */
+    CR_ERR_P_RANGE = -7,         /* p < 1 */
+    CR_ERR_TOO_LARGE = -8,       /* state space exceeds device memory or 32-bit config ids */
+    CR_ERR_DEVICE = -9,          /* HIP runtime error */
+    CR_ERR_ORDER = -10,          /* step called out of order */
+    CR_ERR_UNSUPPORTED = -11,    /* rule combination not implemented on the device */
+    CR_ERR_NO_DEVICE = -12       /* no HIP device visible */
+};
+
+/* Solver variants: which reference solver translation unit is replaced. */
+enum cr_variant {
+    CR_VARIANT_COLUMNS = 1, /* solvers/k_cops_visibility.cpp (2p columns; p = 1 = plain game) */
+    CR_VARIANT_MACRO = 2    /* solvers/k_cops_visibility_v2.cpp (one column, p-step macro turns) */
+};
+
+typedef struct cr_solver cr_solver;
+
+typedef struct {
+    int32_t n;             /* AuxGraph::N */
+    int32_t k;             /* AuxGraph::k */
+    int32_t p;
+    int32_t columns;       /* AuxGraph::columns (2p, or 1 for the macro variant) */
+    int32_t variant;
+    int32_t max_degree;
+    uint64_t config_count; /* AuxGraph::configCount */
+    uint64_t num_states;   /* AuxGraph::numStates */
+    uint64_t device_bytes; /* device memory held by this solver */
+} cr_info;
+
+typedef struct {
+    int32_t win;              /* 1 = WIN, 0 = LOSS */
+    int32_t plys;             /* overallMinWorstCase (valid when win) */
+    uint64_t best_cid;        /* lowest cId achieving it (valid when win) */
+    uint8_t positions[CR_MAX_K]; /* configs[best_cid*k .. +k) */
+} cr_verdict;
+
+typedef struct {
+    uint64_t base_captures; /* "Initialized n base captures." */
+    uint64_t won_states;    /* marked states of every column */
+    int32_t n_passes;       /* reference passes (last one reports 0) */
+    int32_t n_levels;       /* device levels / passes executed */
+    double init_ms;         /* device time of initializeCaptures */
+    double loop_ms;         /* device time of mainLoop */
+} cr_stats;
+
+/* ---- lifetime -------------------------------------------------------- */
+int cr_solver_create(int device, cr_solver** out);
+void cr_solver_destroy(cr_solver* s);
+/* Message of the last failure on this thread ("" if none). */
+const char* cr_last_error(void);
+
+/* ---- graph input ----------------------------------------------------- */
+/* Graph::Graph(const char*) parsing, lib/src/Graph.cpp:15-79: node count from
+ * the first line's width, '0'/'1' cells, '-' ends the matrix. Returns N (0 for
+ * an empty or malformed text) and fills adj[N*N] when N <= cap. */
+int32_t cr_parse_matrix(const uint8_t* text, uint64_t len, uint8_t* adj, int32_t cap);
+
+/* loadGraphFile(filename, k, p), solvers/k_cops_visibility.cpp:38-57
+ * (Graph + AdjacencyList::constructFrom, AdjacencyList.cpp:33-72). */
+int cr_load_graph_file(cr_solver* s, const char* filename, int32_t k, int32_t p);
+/* Same step from an in-memory N*N 0/1 matrix (row u = moves out of u). */
+int cr_load_graph_matrix(cr_solver* s, int32_t n, const uint8_t* adj, int32_t k, int32_t p);
+/* Selects the solver TU being replaced (default CR_VARIANT_COLUMNS). */
+int cr_set_variant(cr_solver* s, int32_t variant);
+
+/* ---- the six solver steps -------------------------------------------- */
+/* buildAuxGraph(), k_cops_visibility.cpp:59-83 -> AuxGraph::constructFrom
+ * (AuxGraph.h:59-79): configuration count, device state space. Configurations
+ * and team moves are not materialised; the device ranks and unranks sorted
+ * cop tuples in closed form (AuxGraph.h:117-177 order). */
+int cr_build_aux_graph(cr_solver* s);
+/* initializeCaptures(), k_cops_visibility.cpp:85-127 (v2 :200-224). */
+int cr_initialize_captures(cr_solver* s, uint64_t* base_captures);
+/* mainLoop(), k_cops_visibility.cpp:129-299 (v2 :343-565). pass_counts
+ * receives the per-pass "Found n new winning states." values (at most
+ * max_passes of them); n_passes the number of passes the reference prints. */
+int cr_main_loop(cr_solver* s, uint64_t* pass_counts, int32_t max_passes, int32_t* n_passes);
+/* findFinalResult(), k_cops_visibility.cpp:301-354 (v2 :567-619). */
+int cr_find_final_result(cr_solver* s, cr_verdict* out);
+/* outputData() -> CacheManager::saveAuxGraph, CacheManager.h:28-75: writes the
+ * 96-byte header, configs, transition heads, transitions (nextId*N) and
+ * states to path. */
+int cr_save_aux_graph(cr_solver* s, const char* path);
+
+/* ---- accessors ------------------------------------------------------- */
+int cr_get_info(const cr_solver* s, cr_info* out);
+int cr_get_stats(const cr_solver* s, cr_stats* out);
+/* AuxGraph::states bytes [offset, offset+count) (AuxGraph.h:32-33, 96-98). */
+int cr_copy_states(cr_solver* s, uint8_t* dst, uint64_t offset, uint64_t count);
+/* AuxGraph::configs rows [first, first+count), k bytes each (AuxGraph.h:28). */
+int cr_copy_configs(cr_solver* s, uint8_t* dst, uint64_t first, uint64_t count);
+/* Depth-level histogram of a solved column-0/1 table: out[2*d + t] = states of
+ * column t won at depth d (d < 127). Only for the p = 1 plain game. */
+int cr_level_counts(cr_solver* s, uint64_t* out);
+/* Retro-edge count of the solved p = 1 table (the benchmark's unit of work):
+ * e_cop = sum over won robber-turn states (depth <= 125) of |pred(C')|,
+ * e_rob = sum over won cop-turn states (depth <= 125) of in-degree(v). */
+int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_t* won);
+/* Total team moves sum over C of |T(C)|: the "Total edge pointers" the
+ * reference prints after AuxGraph::createTransitions (AuxGraph.h:274). */
+int cr_count_transitions(cr_solver* s, uint64_t* total);
+/* Per-kernel device time of the last cr_main_loop when timing is enabled:
+ * name_out receives a '\0'-separated list, ms/launches one entry per name. */
+int cr_set_kernel_timing(cr_solver* s, int32_t enabled);
+int cr_kernel_times(cr_solver* s, char* names_out, int32_t names_cap, double* ms,
+                    int64_t* launches, int32_t cap, int32_t* count);
+/* Build-time cache key, CacheManager::buildCacheKey (CacheManager.h:16-26),
+ * without creating the directory. Returns the length written (excluding
+ * '\0') or a negative status. */
+int cr_cache_key(const char* graph_path, int32_t k, int32_t p, int32_t variant, char* out,
+                 int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRSOLVE_H */

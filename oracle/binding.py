@@ -1,0 +1,191 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, the smoke check and the
+CPU-baseline leg of bench.py, always as the checker of the HIP path.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+MAX_ROUND = 127
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        i32, i64, u64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
+        L.or_parse_matrix.argtypes = [u8p, u64, u8p, i32]
+        L.or_parse_matrix.restype = i32
+        L.or_config_count.argtypes = [i32, i32]
+        L.or_config_count.restype = u64
+        L.or_enum_configs.argtypes = [i32, i32, u8p, u64]
+        L.or_enum_configs.restype = i32
+        L.or_rank.argtypes = [u8p, i32, i32]
+        L.or_rank.restype = u64
+        L.or_transitions.argtypes = [i32, u8p, i32, i32, u64p, u64p, u64]
+        L.or_transitions.restype = i64
+        L.or_v1_solve.argtypes = [i32, u8p, i32, i32, i32, i32, u8p, u64p, u64p, i32,
+                                  ctypes.POINTER(i32)]
+        L.or_v1_solve.restype = i32
+        L.or_verdict.argtypes = [i32, i32, i32, u8p, ctypes.POINTER(i64), ctypes.POINTER(i32)]
+        L.or_verdict.restype = i32
+        L.or_v2_solve.argtypes = [i32, u8p, i32, i32, u8p, u64p, u64p, i32, ctypes.POINTER(i32)]
+        L.or_v2_solve.restype = i32
+        L.or_v1_levelsync.argtypes = [i32, u8p, i32, i32, u8p, u64p, u64p, u64p]
+        L.or_v1_levelsync.restype = i32
+        L.or_retro_edges.argtypes = [i32, u8p, i32, u8p, u64p, u64p, u64p]
+        L.or_retro_edges.restype = i32
+        L.or_pass_counts_from_depths.argtypes = [i32, i32, u8p, u64p, i32]
+        L.or_pass_counts_from_depths.restype = i32
+        _lib = L
+    return _lib
+
+
+def _p8(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def _p64(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def parse_matrix(text: bytes):
+    """Graph.cpp:15-79. Returns (n, adj uint8[n, n])."""
+    buf = np.frombuffer(text, dtype=np.uint8).copy() if text else np.zeros(1, np.uint8)
+    n = lib().or_parse_matrix(_p8(buf), len(text), None, 0)
+    if n == 0:
+        return 0, np.zeros((0, 0), np.uint8)
+    adj = np.zeros((n, n), np.uint8)
+    lib().or_parse_matrix(_p8(buf), len(text), _p8(adj), n)
+    return n, adj
+
+
+def load_matrix(path):
+    with open(path, "rb") as f:
+        return parse_matrix(f.read())
+
+
+def config_count(n, k):
+    return int(lib().or_config_count(n, k))
+
+
+def configs(n, k):
+    C = config_count(n, k)
+    out = np.zeros((C, k), np.uint8)
+    assert lib().or_enum_configs(n, k, _p8(out), C) == 0
+    return out
+
+
+def rank(tup, n):
+    t = np.ascontiguousarray(np.asarray(tup, np.uint8))
+    return int(lib().or_rank(_p8(t), n, len(t)))
+
+
+def transitions(adj, k, cop_stay=0):
+    """AuxGraph.h:179-275: (heads uint64[C+1], transitions uint64[...] = nextId*N)."""
+    adj = np.ascontiguousarray(adj, np.uint8)
+    n = adj.shape[0]
+    C = config_count(n, k)
+    heads = np.zeros(C + 1, np.uint64)
+    cap = 1 << 16
+    while True:
+        trans = np.zeros(cap, np.uint64)
+        tot = lib().or_transitions(n, _p8(adj), k, cop_stay, _p64(heads), _p64(trans), cap)
+        if tot >= 0:
+            return heads, trans[:tot].copy()
+        need = int(heads[-1])
+        if need == 0:
+            raise RuntimeError("or_transitions failed")
+        cap = need
+
+
+class Solved:
+    def __init__(self, states, base_caps, pass_counts, columns, n, k):
+        self.states = states
+        self.base_caps = base_caps
+        self.pass_counts = pass_counts
+        self.columns = columns
+        self.n = n
+        self.k = k
+
+    def verdict(self):
+        best = ctypes.c_int64()
+        plys = ctypes.c_int32()
+        lib().or_verdict(self.n, self.k, self.columns, _p8(self.states), ctypes.byref(best),
+                         ctypes.byref(plys))
+        return int(best.value), int(plys.value)
+
+
+def v1_solve(adj, k, p=1, cop_stay=0, robber_stay=0, max_passes=8192):
+    adj = np.ascontiguousarray(adj, np.uint8)
+    n = adj.shape[0]
+    C = config_count(n, k)
+    states = np.zeros(C * n * 2 * p, np.uint8)
+    base = ctypes.c_uint64()
+    pc = np.zeros(max_passes, np.uint64)
+    npass = ctypes.c_int32()
+    rc = lib().or_v1_solve(n, _p8(adj), k, p, cop_stay, robber_stay, _p8(states),
+                           ctypes.byref(base), _p64(pc), max_passes, ctypes.byref(npass))
+    if rc != 0:
+        raise RuntimeError("or_v1_solve failed")
+    return Solved(states, int(base.value), [int(x) for x in pc[:npass.value]], 2 * p, n, k)
+
+
+def v2_solve(adj, k, p, max_passes=8192):
+    adj = np.ascontiguousarray(adj, np.uint8)
+    n = adj.shape[0]
+    C = config_count(n, k)
+    states = np.zeros(C * n, np.uint8)
+    base = ctypes.c_uint64()
+    pc = np.zeros(max_passes, np.uint64)
+    npass = ctypes.c_int32()
+    rc = lib().or_v2_solve(n, _p8(adj), k, p, _p8(states), ctypes.byref(base), _p64(pc),
+                           max_passes, ctypes.byref(npass))
+    if rc != 0:
+        raise RuntimeError("or_v2_solve failed")
+    return Solved(states, int(base.value), [int(x) for x in pc[:npass.value]], 1, n, k)
+
+
+def v1_levelsync(adj, k, nthreads=1):
+    """Returns (states, level_counts[127, 2], e_cop, e_rob)."""
+    adj = np.ascontiguousarray(adj, np.uint8)
+    n = adj.shape[0]
+    C = config_count(n, k)
+    states = np.zeros(C * n * 2, np.uint8)
+    lv = np.zeros(2 * MAX_ROUND, np.uint64)
+    ec = ctypes.c_uint64()
+    er = ctypes.c_uint64()
+    rc = lib().or_v1_levelsync(n, _p8(adj), k, nthreads, _p8(states), _p64(lv), ctypes.byref(ec),
+                               ctypes.byref(er))
+    if rc != 0:
+        raise RuntimeError("or_v1_levelsync failed")
+    return states, lv.reshape(MAX_ROUND, 2), int(ec.value), int(er.value)
+
+
+def retro_edges(adj, k, states):
+    adj = np.ascontiguousarray(adj, np.uint8)
+    ec, er, w = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    assert lib().or_retro_edges(adj.shape[0], _p8(adj), k, _p8(states), ctypes.byref(ec),
+                                ctypes.byref(er), ctypes.byref(w)) == 0
+    return int(ec.value), int(er.value), int(w.value)
+
+
+def pass_counts_from_depths(n, k, states, max_passes=8192):
+    pc = np.zeros(max_passes, np.uint64)
+    np_ = lib().or_pass_counts_from_depths(n, k, _p8(states), _p64(pc), max_passes)
+    return [int(x) for x in pc[:np_]]

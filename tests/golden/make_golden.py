@@ -1,0 +1,160 @@
+"""Write the golden fixtures under tests/golden/ from the reference itself.
+
+Runs in the development container only (it needs the reference tree and the
+reference binaries built by oracle/build_ref.sh). What it writes is data:
+
+* matrices/*.txt      the reference's input graphs (assets/matrices), verbatim.
+* csv/*.csv           the reference's recorded regression results
+                      (console_outputs/test_results*.csv), verbatim.
+* ref/*.npz           per (solver, graph, k, p): the sections of the cache blob
+                      the compiled reference writes (header, configs, heads,
+                      transitions, states), or only their SHA-256 for the
+                      large cases, plus the facts printed on stdout.
+* ref_index.json      the stdout facts of every case (pass counts, base
+                      captures, edge pointers, verdict lines, blob hash).
+
+Usage: python tests/golden/make_golden.py [--with-slow]
+"""
+import argparse
+import hashlib
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+REF = os.environ.get("CR_REFERENCE_DIR", "/root/reference")
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+BIN = os.path.join(REPO, "oracle", "_ref")
+
+V1_CASES = [
+    # (graph, k, p) -- p = 1 is the plain full-information solver
+    ("peterson", 1, 1), ("peterson", 2, 1), ("peterson", 3, 1),
+    ("grid36", 2, 1), ("grid36", 3, 1), ("planar32", 3, 1), ("dismantleable48", 2, 1),
+    ("line100", 1, 1), ("tree127", 1, 1), ("cycle100", 2, 1), ("dodecahedron", 3, 1),
+    ("robertson", 2, 1), ("outerplanar8", 3, 1), ("test", 2, 1), ("cycle3", 1, 1),
+    ("line3", 2, 1), ("rooks", 2, 1), ("planar8", 2, 1), ("grid9", 3, 1),
+    # p >= 2 (2p columns)
+    ("peterson", 3, 2), ("peterson", 3, 3), ("grid9", 2, 2), ("grid9", 2, 3),
+    ("line10", 2, 3), ("cycle10", 3, 2), ("dismantleable19", 2, 2), ("planar32", 2, 2),
+    ("outerplanar32", 2, 3), ("line5", 1, 2), ("tree7", 1, 3),
+]
+V1_BIG = [("scotlandyard-all", 2, 1), ("scotlandyard-yellow", 2, 1)]
+V2_CASES = [
+    ("grid9", 2, 2), ("grid9", 2, 3), ("cycle10", 2, 3), ("line10", 2, 3),
+    ("peterson", 3, 2), ("peterson", 3, 3), ("dismantleable19", 2, 3), ("planar32", 2, 2),
+    ("tree7", 2, 3), ("rooks", 2, 3), ("outerplanar32", 2, 3), ("peterson", 3, 1),
+    ("grid36", 2, 1), ("line5", 1, 2), ("cycle5", 1, 3),
+]
+# Full-size limited-visibility workload (8x8 grid + 16 chords, k=2, p=3):
+# about 80 s of reference time, so only with --with-slow.
+V2_SLOW = [("grid8x8_chords16_s1", 2, 3)]
+INLINE_LIMIT = 4 << 20  # store arrays for blobs up to 4 MiB, hashes above
+
+
+def stdout_facts(text):
+    facts = {
+        "passes": [int(m) for m in re.findall(r"Pass \d+: Found (\d+) new", text)],
+        "base_caps": None, "edge_pointers": None, "result": None, "positions": None,
+        "plys": None, "cycles": None, "cache_hit": "Loaded AuxGraph from cache" in text,
+    }
+    m = re.search(r"Initialized (\d+) base captures", text)
+    facts["base_caps"] = int(m.group(1)) if m else None
+    m = re.search(r"Total edge pointers: (\d+)", text)
+    facts["edge_pointers"] = int(m.group(1)) if m else None
+    m = re.search(r"RESULT:\s+(WIN|LOSS)", text)
+    facts["result"] = m.group(1) if m else None
+    m = re.search(r"Optimal Cop Start Positions:\s*\(([^)]*)\)", text)
+    facts["positions"] = m.group(1).strip() if m else None
+    m = re.search(r"Capture Time:\s*(\d+)\s*plys(?: \((\d+) full cycles\))?", text)
+    if m:
+        facts["plys"] = int(m.group(1))
+        facts["cycles"] = int(m.group(2)) if m.group(2) is not None else None
+    return facts
+
+
+def run_ref(exe, matrix, k, p):
+    work = tempfile.mkdtemp(prefix="crgold_")
+    try:
+        proc = subprocess.run([os.path.join(BIN, exe), matrix, str(k), str(p)], cwd=work,
+                              capture_output=True, text=True, check=True)
+        files = os.listdir(os.path.join(work, "cache"))
+        assert len(files) == 1, files
+        with open(os.path.join(work, "cache", files[0]), "rb") as f:
+            blob = f.read()
+        return proc.stdout, files[0], blob
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def split_blob(blob):
+    h = np.frombuffer(blob[:96], np.uint64).copy()
+    o1, o2, o3, o4 = (int(x) for x in h[8:12])
+    return {
+        "header": h,
+        "configs": np.frombuffer(blob[o1:o2], np.uint8).copy(),
+        "heads": np.frombuffer(blob[o2:o3], np.uint64).copy(),
+        "transitions": np.frombuffer(blob[o3:o4], np.uint64).copy(),
+        "states": np.frombuffer(blob[o4:], np.uint8).copy(),
+    }
+
+
+def record(index, exe, tag, graph, k, p, mdir):
+    out, fname, blob = run_ref(exe, os.path.join(mdir, graph + ".txt"), k, p)
+    name = f"{tag}__{graph}__k{k}__p{p}"
+    facts = stdout_facts(out)
+    facts.update({"solver": tag, "graph": graph, "k": k, "p": p, "cache_file": fname,
+                  "blob_sha256": hashlib.sha256(blob).hexdigest(), "blob_bytes": len(blob)})
+    sec = split_blob(blob)
+    facts["states_sha256"] = hashlib.sha256(sec["states"].tobytes()).hexdigest()
+    if len(blob) <= INLINE_LIMIT:
+        np.savez_compressed(os.path.join(HERE, "ref", name + ".npz"), **sec)
+        facts["arrays"] = name + ".npz"
+    else:
+        facts["arrays"] = None
+    index[name] = facts
+    print(f"{name}: {facts['result']} {facts['positions']} plys={facts['plys']} "
+          f"passes={len(facts['passes'])} blob={len(blob)}", flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--with-slow", action="store_true")
+    args = ap.parse_args()
+    if not os.path.exists(os.path.join(BIN, "k_cops_visibility")):
+        sys.exit("reference binaries missing: run oracle/build_ref.sh first")
+    mdir = os.path.join(HERE, "matrices")
+    os.makedirs(mdir, exist_ok=True)
+    os.makedirs(os.path.join(HERE, "csv"), exist_ok=True)
+    os.makedirs(os.path.join(HERE, "ref"), exist_ok=True)
+    for fn in sorted(os.listdir(os.path.join(REF, "assets", "matrices"))):
+        shutil.copyfile(os.path.join(REF, "assets", "matrices", fn), os.path.join(mdir, fn))
+    for fn in ("test_results.csv", "test_results_v2.csv", "test_results_affected.csv"):
+        shutil.copyfile(os.path.join(REF, "console_outputs", fn), os.path.join(HERE, "csv", fn))
+    # generated workloads (written by the package's own seeded generators)
+    sys.path.insert(0, REPO)
+    from copsandrobbers_amd import workloads
+    workloads.write_all(mdir)
+    idx_path = os.path.join(HERE, "ref_index.json")
+    index = {}
+    if os.path.exists(idx_path):
+        with open(idx_path) as f:
+            index = json.load(f)
+    for g, k, p in V1_CASES + V1_BIG:
+        record(index, "k_cops_visibility", "v1", g, k, p, mdir)
+    for g, k, p in V2_CASES:
+        record(index, "k_cops_visibility_v2", "v2", g, k, p, mdir)
+    if args.with_slow:
+        for g, k, p in V2_SLOW:
+            record(index, "k_cops_visibility_v2", "v2", g, k, p, mdir)
+    with open(idx_path, "w") as f:
+        json.dump(index, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

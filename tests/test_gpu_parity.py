@@ -1,0 +1,180 @@
+"""Parity of the HIP solver (through the C ABI) with the reference and the oracle.
+
+Expected values: the compiled reference's cache blobs and stdout facts
+(tests/golden/ref*), the reference's recorded CSV rows, and the CPU oracle on
+the same inputs. Integer work: every comparison is exact.
+"""
+import csv
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, REPO, matrix
+
+import copsandrobbers_amd as cra
+
+pytestmark = pytest.mark.gpu
+
+with open(os.path.join(GOLDEN, "ref_index.json")) as f:
+    INDEX = json.load(f)
+
+P1 = sorted(k for k, v in INDEX.items() if v["solver"] == "v1" and v["p"] == 1)
+
+
+def say(*args):
+    print(*args, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def solver():
+    s = cra.Solver(0)
+    yield s
+    s.close()
+
+
+def run_steps(solver, path, k, p=1, variant=cra.VARIANT_COLUMNS):
+    solver.load_graph_file(path, k, p, variant)
+    solver.build_aux_graph()
+    base = solver.initialize_captures()
+    passes = solver.main_loop()
+    verdict = solver.find_final_result()
+    return base, passes, verdict
+
+
+def positions(v):
+    return ", ".join(str(x) for x in v["positions"])
+
+
+@pytest.mark.parametrize("name", P1)
+def test_plain_tables_match_reference(solver, name):
+    e = INDEX[name]
+    say("case", name)
+    base, passes, verdict = run_steps(solver, matrix(e["graph"]), e["k"])
+    states = solver.states()
+    if e["arrays"]:
+        g = np.load(os.path.join(GOLDEN, "ref", e["arrays"]))
+        assert np.array_equal(states, g["states"])
+    assert hashlib.sha256(states.tobytes()).hexdigest() == e["states_sha256"]
+    assert passes == e["passes"]
+    assert base == e["base_caps"]
+    assert solver.count_transitions() == e["edge_pointers"]
+    if e["result"] == "WIN":
+        assert verdict is not None
+        assert positions(verdict) == e["positions"]
+        assert verdict["plys"] == e["plys"]
+    else:
+        assert verdict is None
+
+
+@pytest.mark.parametrize("name", ["v1__peterson__k3__p1", "v1__grid36__k2__p1", "v1__test__k2__p1",
+                                  "v1__line100__k1__p1"])
+def test_cache_blob_matches_reference_bytes(solver, name, tmp_path):
+    e = INDEX[name]
+    say("blob", name)
+    run_steps(solver, matrix(e["graph"]), e["k"])
+    out = tmp_path / "blob.bin"
+    solver.output_data(out)
+    data = out.read_bytes()
+    assert len(data) == e["blob_bytes"]
+    assert hashlib.sha256(data).hexdigest() == e["blob_sha256"]
+
+
+def test_levels_and_retro_edges_match_oracle(solver, oracle):
+    for graph, k in (("grid36", 2), ("planar32", 3), ("dodecahedron", 3), ("scotlandyard-all", 2)):
+        say("retro", graph, k)
+        n, adj = oracle.load_matrix(matrix(graph))
+        run_steps(solver, matrix(graph), k)
+        states, levels, e_cop, e_rob = oracle.v1_levelsync(adj, k, nthreads=8)
+        assert np.array_equal(solver.states(), states)
+        assert np.array_equal(solver.level_counts(), levels)
+        got = solver.retro_edges()
+        assert got == (e_cop, e_rob, int((states & 1).sum()))
+
+
+def test_er200_k2_matches_oracle(solver, oracle):
+    path = matrix("er200_p0.03_s1")
+    n, adj = oracle.load_matrix(path)
+    say("er200 k=2")
+    base, passes, verdict = run_steps(solver, path, 2)
+    states, levels, e_cop, e_rob = oracle.v1_levelsync(adj, 2, nthreads=8)
+    assert np.array_equal(solver.states(), states)
+    assert passes == oracle.pass_counts_from_depths(n, 2, states)
+    best, plys = oracle.Solved(states, base, passes, 2, n, 2).verdict()
+    assert (verdict is None) == (best == -1)
+
+
+def csv_rows(fn):
+    with open(os.path.join(GOLDEN, "csv", fn)) as f:
+        return [r for r in csv.DictReader(f) if r["status"] == "ok" and r["p"] == "1"]
+
+
+ROWS = csv_rows("test_results.csv")
+
+
+def test_recorded_csv_rows_plain_game(solver):
+    for row in ROWS:
+        say("csv", row["graph"], row["k"])
+        base, passes, verdict = run_steps(solver, matrix(row["graph"]), int(row["k"]))
+        assert str(base) == row["base_caps"]
+        assert str(len(passes)) == row["passes"]
+        if row["result"] == "WIN":
+            assert positions(verdict) == row["positions"]
+            assert str(verdict["plys"]) == row["capture_plys"]
+        else:
+            assert verdict is None
+
+
+def test_cli_reproduces_recorded_rows(tmp_path):
+    exe = os.path.join(REPO, "copsandrobbers_amd", "bin", "k_cops_visibility")
+    from copsandrobbers_amd import harness
+    for row in ROWS[::3]:
+        say("cli", row["graph"], row["k"])
+        res = harness.run_one(exe, matrix(row["graph"]), int(row["k"]), 1, timeout=120, cwd=str(tmp_path))
+        assert res["status"] == "ok", res
+        for col in ("result", "capture_plys", "passes", "base_caps", "positions"):
+            assert str(res[col]) == row[col], (row["graph"], col)
+
+
+def test_error_paths(solver, tmp_path):
+    say("errors")
+    _, adj = cra.parse_matrix(open(matrix("peterson"), "rb").read())
+    cases = [
+        (lambda: solver.load_graph_matrix(adj, 0), -6),
+        (lambda: solver.load_graph_matrix(adj, 9), -6),
+        (lambda: solver.load_graph_matrix(adj, 2, 0), -7),
+        (lambda: solver.load_graph_matrix(np.zeros((0, 0), np.uint8), 1), -3),
+        (lambda: solver.load_graph_matrix(np.ones((256, 256), np.uint8), 1), -4),
+    ]
+    iso = adj.copy()
+    iso[3, :] = 0
+    iso[:, 3] = 0
+    cases.append((lambda: solver.load_graph_matrix(iso, 1), -5))
+    empty = tmp_path / "empty.txt"
+    empty.write_bytes(b"")
+    cases.append((lambda: solver.load_graph_file(str(empty), 1), -3))
+    cases.append((lambda: solver.load_graph_file(str(tmp_path / "missing.txt"), 1), -3))
+    for fn, code in cases:
+        with pytest.raises(cra.SolverError) as exc:
+            fn()
+        assert exc.value.code == code
+    fresh = cra.Solver(0)
+    with pytest.raises(cra.SolverError) as exc:
+        fresh.build_aux_graph()
+    assert exc.value.code == -10
+    fresh.close()
+
+
+def test_repeat_solves_are_identical(solver):
+    """Solving twice in one solver (re-initialised tables) gives equal bytes."""
+    say("repeat")
+    run_steps(solver, matrix("grid36"), 3)
+    a = solver.states()
+    solver.initialize_captures()
+    solver.main_loop()
+    assert np.array_equal(solver.states(), a)
