@@ -80,6 +80,20 @@ def cpu_port(stem, k, threads):
             "sample": f"multithreaded level-synchronous oracle, full workload, {dt:.2f} s"}, states
 
 
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of known size: FETCH_SIZE/WRITE_SIZE of this
+    kernel give the factor between counter units and bytes."""
+    import torch
+    x = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "copies": 3}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,7 +103,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-port", type=int, default=0, help="also time the oracle port on N threads")
     ap.add_argument("--check", action="store_true", help="compare the table with the oracle port")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (rocprof counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(json.dumps({"error": "this solver runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "design-only)"}))

@@ -193,55 +193,121 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // ---------------------------------------------------------------------------
-// init: reset every state, seed captures (k_cops_visibility.cpp:85-127)
+// init: reset every state, seed captures (k_cops_visibility.cpp:85-127) and
+// resolve depth 1 in closed form. With no stay moves and no vertex without a
+// move, a cop-turn state (C, r) is won at depth 1 iff some cop of C has r as
+// an out-neighbour (that cop steps onto the robber; the others move anywhere),
+// and a robber-turn state iff every move of r lands on a cop of C. This is the
+// push of level 0 (every capture state to all of its predecessors) done per
+// row without touching any other row.
 // ---------------------------------------------------------------------------
-template <int K>
+template <int K, int W>
 __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     load_image(lds, gv.image, gv.image_bytes);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const int lane = threadIdx.x & 63;
-    const int W = gv.W;
+    uint64_t om[W][W];  // out-masks of robber vertices lane + 64 j
+    bool valid[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const int r = lane + 64 * j;
+        valid[j] = r < gv.n;
+#pragma unroll
+        for (int q = 0; q < W; ++q) om[j][q] = valid[j] ? a.out_mask[r * 4 + q] : 0ull;
+    }
     const uint32_t nbatch = (gv.C + 63) / 64;
-    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    unsigned long long caps = 0;
-    for (uint32_t b = gw; b < nbatch; b += gridDim.x * kWavesPerBlock) {
+    const uint8_t w1b = won_byte(1);
+    unsigned long long caps = 0, cop1 = 0, rob1 = 0;
+    for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
+         b += gridDim.x * kWavesPerBlock) {
         const uint32_t row = b * 64 + lane;
-        uint64_t cm[4] = {0, 0, 0, 0};
+        uint64_t cm[W], am[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) cm[q] = am[q] = 0;
         if (row < gv.C) {
             uint32_t cops[K];
             unrank<K>(row, gv, binom, cops);
 #pragma unroll
-            for (int i = 0; i < K; ++i) cm[cops[i] >> 6] |= 1ull << (cops[i] & 63);
-            for (int w = 0; w < W; ++w) {
-                const uint64_t o = static_cast<uint64_t>(row) * W + w;
-                a.won0[o] = cm[w];
-                a.won1[o] = cm[w];
-                a.fr1[o] = cm[w];
-                a.nx0[o] = 0;
+            for (int i = 0; i < K; ++i) {
+#pragma unroll
+                for (int q = 0; q < W; ++q) {
+                    if ((cops[i] >> 6) == static_cast<uint32_t>(q)) cm[q] |= 1ull << (cops[i] & 63);
+                    am[q] |= a.out_mask[cops[i] * 4 + q];
+                }
             }
-            a.list0_nxt[row] = row;  // level 0: every configuration holds captures
-            a.list1_nxt[row] = row;
-            caps += __popcll(cm[0]) + __popcll(cm[1]) + __popcll(cm[2]) + __popcll(cm[3]);
         }
+        uint64_t my_r1[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) my_r1[q] = 0;
         const uint32_t rows_here = min(64u, gv.C - b * 64);
         for (uint32_t i = 0; i < rows_here; ++i) {
             const uint32_t rr = b * 64 + i;
-            uint64_t m[4];
+            uint64_t c[W], adj[W], r1[W];
 #pragma unroll
-            for (int w = 0; w < 4; ++w) m[w] = __shfl(cm[w], static_cast<int>(i));
+            for (int q = 0; q < W; ++q) {
+                c[q] = __shfl(cm[q], static_cast<int>(i));
+                adj[q] = __shfl(am[q], static_cast<int>(i));
+            }
             uint16_t* trow = reinterpret_cast<uint16_t*>(a.table + static_cast<uint64_t>(rr) * gv.n * 2);
-            for (int r = lane; r < gv.n; r += 64) {
-                const bool cap = (m[r >> 6] >> (r & 63)) & 1;
-                trow[r] = cap ? static_cast<uint16_t>(0x0101) : static_cast<uint16_t>(kUnmarked * 0x0101);
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                const bool cap = (c[j] >> lane) & 1;
+                bool forced = valid[j] && !cap;  // every robber move lands on a cop
+#pragma unroll
+                for (int q = 0; q < W; ++q) forced = forced && ((om[j][q] & ~c[q]) == 0);
+                r1[j] = __ballot(forced);
+                if (valid[j]) {
+                    const uint8_t b0 = cap ? won_byte(0) : (((adj[j] >> lane) & 1) ? w1b : kUnmarked);
+                    const uint8_t b1 = cap ? won_byte(0) : (forced ? w1b : kUnmarked);
+                    trow[lane + 64 * j] = static_cast<uint16_t>(b0 | (b1 << 8));
+                }
+            }
+            if (lane == static_cast<int>(i)) {
+#pragma unroll
+                for (int q = 0; q < W; ++q) my_r1[q] = r1[q];
             }
         }
+        bool has0 = false, has1 = false;
+        if (row < gv.C) {
+            const uint64_t o = static_cast<uint64_t>(row) * W;
+#pragma unroll
+            for (int q = 0; q < W; ++q) {
+                const uint64_t c1 = am[q] & ~cm[q];
+                a.won0[o + q] = cm[q] | c1;
+                a.won1[o + q] = cm[q] | my_r1[q];
+                a.fr1[o + q] = my_r1[q];
+                a.nx0[o + q] = 0;
+                caps += __popcll(cm[q]);
+                cop1 += __popcll(c1);
+                rob1 += __popcll(my_r1[q]);
+                has0 |= c1 != 0;
+                has1 |= my_r1[q] != 0;
+            }
+        }
+        const uint64_t b0 = __ballot(has0), b1 = __ballot(has1);
+        uint32_t p0 = 0, p1 = 0;
+        if (lane == 0) {
+            if (b0) p0 = atomicAdd(&a.ctl->cnt0[a.cur ^ 1], static_cast<uint32_t>(__popcll(b0)));
+            if (b1) p1 = atomicAdd(&a.ctl->cnt1[a.cur ^ 1], static_cast<uint32_t>(__popcll(b1)));
+        }
+        p0 = __shfl(p0, 0);
+        p1 = __shfl(p1, 0);
+        if (has0) a.list0_nxt[p0 + __popcll(b0 & lanemask_lt())] = row;
+        if (has1) a.list1_nxt[p1 + __popcll(b1 & lanemask_lt())] = row;
     }
-    // wave-reduce the capture count
-    for (int off = 32; off > 0; off >>= 1) caps += __shfl_xor(caps, off);
-    if (lane == 0 && caps) {
-        atomicAdd(&a.ctl->lvl[0][0], caps);
-        atomicAdd(&a.ctl->lvl[0][1], caps);
+    for (int off = 32; off > 0; off >>= 1) {
+        caps += __shfl_xor(caps, off);
+        cop1 += __shfl_xor(cop1, off);
+        rob1 += __shfl_xor(rob1, off);
+    }
+    if (lane == 0) {
+        if (caps) {
+            atomicAdd(&a.ctl->lvl[0][0], caps);
+            atomicAdd(&a.ctl->lvl[0][1], caps);
+        }
+        if (cop1) atomicAdd(&a.ctl->lvl[1][0], cop1);
+        if (rob1) atomicAdd(&a.ctl->lvl[1][1], rob1);
     }
 }
 
@@ -317,16 +383,25 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
     unsigned long long found = 0;
     for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
          b += gridDim.x * kWavesPerBlock) {
-        const uint32_t mine = b * 64 + lane < count ? a.list0_cur[b * 64 + lane] : 0u;
+        // each lane loads one row of the batch; the row loop then only shuffles
+        const bool live = b * 64 + lane < count;
+        const uint32_t mine = live ? a.list0_cur[b * 64 + lane] : 0u;
+        uint64_t lw0[W], lw1[W], lnew[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            lw0[q] = live ? a.won0[static_cast<uint64_t>(mine) * W + q] : 0ull;
+            lw1[q] = live ? a.won1[static_cast<uint64_t>(mine) * W + q] : 0ull;
+            lnew[q] = 0;
+        }
         const uint32_t rows_here = min(64u, count - b * 64);
-        bool append = false;
         for (uint32_t i = 0; i < rows_here; ++i) {
             const uint32_t row = __shfl(mine, static_cast<int>(i));
-            const uint64_t o = static_cast<uint64_t>(row) * W;
             uint64_t w0[W], w1[W], nw[W];
 #pragma unroll
-            for (int q = 0; q < W; ++q) { w0[q] = a.won0[o + q]; w1[q] = a.won1[o + q]; }
-            uint32_t pop = 0;
+            for (int q = 0; q < W; ++q) {
+                w0[q] = __shfl(lw0[q], static_cast<int>(i));
+                w1[q] = __shfl(lw1[q], static_cast<int>(i));
+            }
 #pragma unroll
             for (int j = 0; j < W; ++j) {
                 bool ok = valid[j] && !((w1[j] >> lane) & 1);
@@ -334,17 +409,26 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
                 for (int q = 0; q < W; ++q) ok = ok && ((om[j][q] & ~w0[q]) == 0);
                 nw[j] = __ballot(ok);
                 if (ok) a.table[(static_cast<uint64_t>(row) * gv.n + lane + 64 * j) * 2 + 1] = nb;
-                pop += __popcll(nw[j]);
             }
+            if (lane == static_cast<int>(i)) {
+#pragma unroll
+                for (int q = 0; q < W; ++q) lnew[q] = nw[q];
+            }
+        }
+        bool append = false;
+        if (live) {
+            const uint64_t o = static_cast<uint64_t>(mine) * W;
+            uint32_t pop = 0;
+#pragma unroll
+            for (int q = 0; q < W; ++q) pop += __popcll(lnew[q]);
             if (pop) {
 #pragma unroll
-                for (int q = 0; q < W; ++q)
-                    if (lane == q) {
-                        a.won1[o + q] = w1[q] | nw[q];
-                        a.fr1[o + q] = nw[q];
-                    }
+                for (int q = 0; q < W; ++q) {
+                    a.won1[o + q] = lw1[q] | lnew[q];
+                    a.fr1[o + q] = lnew[q];
+                }
+                append = true;
                 found += pop;
-                if (lane == static_cast<int>(i)) append = true;
             }
         }
         const uint64_t bal = __ballot(append);
@@ -355,6 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
             if (append) a.list1_nxt[pos + __popcll(bal & lanemask_lt())] = mine;
         }
     }
+    for (int off = 32; off > 0; off >>= 1) found += __shfl_xor(found, off);
     if (lane == 0 && found) atomicAdd(&a.ctl->lvl[a.depth_new][1], found);
 }
 
@@ -370,42 +455,44 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
     for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
          b += gridDim.x * kWavesPerBlock) {
         const uint32_t row = b * 64 + lane;
+        const uint64_t o = static_cast<uint64_t>(row) * W;
         uint64_t x[W];
         bool any = false;
 #pragma unroll
         for (int q = 0; q < W; ++q) {
-            x[q] = row < gv.C ? a.nx0[static_cast<uint64_t>(row) * W + q] : 0ull;
+            x[q] = row < gv.C ? a.nx0[o + q] : 0ull;
             any |= x[q] != 0;
         }
-        uint64_t bal = __ballot(any);
+        uint64_t nwl[W];
         bool append = false;
-        while (bal) {
-            const int src = __builtin_ctzll(bal);
-            bal &= bal - 1;
-            const uint32_t rr = b * 64 + src;
-            const uint64_t o = static_cast<uint64_t>(rr) * W;
-            uint64_t nw[W];
+        if (any) {  // this lane's row has candidates: settle them
             uint32_t pop = 0;
 #pragma unroll
             for (int q = 0; q < W; ++q) {
-                const uint64_t xv = __shfl(x[q], src);
                 const uint64_t wv = a.won0[o + q];
-                nw[q] = xv & ~wv;
-                pop += __popcll(nw[q]);
-                if (lane == q) {
-                    a.nx0[o + q] = 0;
-                    if (nw[q]) a.won0[o + q] = wv | nw[q];
-                }
+                nwl[q] = x[q] & ~wv;
+                pop += __popcll(nwl[q]);
+                a.nx0[o + q] = 0;
+                if (nwl[q]) a.won0[o + q] = wv | nwl[q];
             }
+            append = pop != 0;
+            found += pop;
+        } else {
 #pragma unroll
-            for (int j = 0; j < W; ++j)
-                if ((nw[j] >> lane) & 1) a.table[(static_cast<uint64_t>(rr) * gv.n + lane + 64 * j) * 2] = nb;
-            if (pop) {
-                found += pop;
-                if (lane == src) append = true;
+            for (int q = 0; q < W; ++q) nwl[q] = 0;
+        }
+        uint64_t bal = __ballot(append);
+        const uint64_t ab = bal;
+        while (bal) {  // table bytes, one row at a time with lanes = robber vertices
+            const int src = __builtin_ctzll(bal);
+            bal &= bal - 1;
+            const uint32_t rr = b * 64 + src;
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                const uint64_t nw = __shfl(nwl[j], src);
+                if ((nw >> lane) & 1) a.table[(static_cast<uint64_t>(rr) * gv.n + lane + 64 * j) * 2] = nb;
             }
         }
-        const uint64_t ab = __ballot(append);
         if (ab) {
             uint32_t pos = 0;
             if (lane == 0) pos = atomicAdd(&a.ctl->cnt0[a.cur ^ 1], static_cast<uint32_t>(__popcll(ab)));
@@ -413,6 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
             if (append) a.list0_nxt[pos + __popcll(ab & lanemask_lt())] = row;
         }
     }
+    for (int off = 32; off > 0; off >>= 1) found += __shfl_xor(found, off);
     if (lane == 0 && found) atomicAdd(&a.ctl->lvl[a.depth_new][0], found);
 }
 
@@ -682,18 +770,39 @@ struct Timer {
     }
 };
 
+// Kernels whose dynamic LDS (the graph image) exceeds the 64 KiB default need
+// the per-function limit raised before launch.
+int raise_lds_limit(const void* fn, uint32_t bytes) {
+    if (bytes <= 64 * 1024) return 0;
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    return 0;
+}
+
+template <int K, int W>
+int launch_init(cr_solver* s, LevelArgs a) {
+    if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_init<K, W>), s->gv.image_bytes)) return rc;
+    hipLaunchKernelGGL((k_init<K, W>), dim3(grid_for((s->C + 63) / 64, 2048)), dim3(kBlock), s->gv.image_bytes,
+                       s->stream, s->gv, a);
+    return 0;
+}
+
 template <int K>
 struct LaunchInit {
     static int run(cr_solver* s, LevelArgs a) {
-        hipLaunchKernelGGL(k_init<K>, dim3(grid_for((s->C + 63) / 64, 2048)), dim3(kBlock), s->gv.image_bytes,
-                           s->stream, s->gv, a);
-        return 0;
+        switch (s->W) {
+            case 1: return launch_init<K, 1>(s, a);
+            case 2: return launch_init<K, 2>(s, a);
+            case 3: return launch_init<K, 3>(s, a);
+            case 4: return launch_init<K, 4>(s, a);
+            default: return fail(CR_ERR_ARG, "row width %d", s->W);
+        }
     }
 };
 
 template <int K>
 struct LaunchPush {
     static int run(cr_solver* s, LevelArgs a, uint32_t rows) {
+        if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_cop_push<K>), s->gv.image_bytes)) return rc;
         hipLaunchKernelGGL(k_cop_push<K>, dim3(grid_for(rows, 2048)), dim3(kBlock), s->gv.image_bytes, s->stream,
                            s->gv, a);
         return 0;
@@ -823,22 +932,6 @@ int upload_graph(cr_solver* s) {
     return 0;
 }
 
-int set_lds_limits(cr_solver* s) {
-    const int bytes = static_cast<int>(s->gv.image_bytes);
-    if (bytes <= 64 * 1024) return 0;
-    const void* fns[] = {
-        reinterpret_cast<const void*>(k_init<1>), reinterpret_cast<const void*>(k_init<2>),
-        reinterpret_cast<const void*>(k_init<3>), reinterpret_cast<const void*>(k_init<4>),
-        reinterpret_cast<const void*>(k_init<5>), reinterpret_cast<const void*>(k_init<6>),
-        reinterpret_cast<const void*>(k_init<7>), reinterpret_cast<const void*>(k_init<8>),
-        reinterpret_cast<const void*>(k_cop_push<1>), reinterpret_cast<const void*>(k_cop_push<2>),
-        reinterpret_cast<const void*>(k_cop_push<3>), reinterpret_cast<const void*>(k_cop_push<4>),
-        reinterpret_cast<const void*>(k_cop_push<5>), reinterpret_cast<const void*>(k_cop_push<6>),
-        reinterpret_cast<const void*>(k_cop_push<7>), reinterpret_cast<const void*>(k_cop_push<8>)};
-    for (const void* f : fns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    return 0;
-}
-
 int do_load(cr_solver* s, int32_t n, const uint8_t* adj, int32_t k, int32_t p) {
     s->release();
     s->stage = kEmpty;
@@ -888,7 +981,6 @@ int do_build(cr_solver* s) {
     if (!(s->d_ctl = dev_alloc<Ctl>(1, s->device_bytes, rc))) return rc;
     if (!s->h_ctl) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_ctl), sizeof(Ctl), hipHostMallocDefault));
     if ((rc = upload_graph(s)) != 0) return rc;
-    if ((rc = set_lds_limits(s)) != 0) return rc;
     s->stage = kBuilt;
     return 0;
 }
@@ -897,8 +989,6 @@ int do_build(cr_solver* s) {
 int reset_ctl(cr_solver* s) {
     std::memset(s->h_ctl, 0, sizeof(Ctl));
     s->h_ctl->verdict = ~0ull;
-    s->h_ctl->cnt0[0] = static_cast<uint32_t>(s->C);  // level 0: every row holds captures
-    s->h_ctl->cnt1[0] = static_cast<uint32_t>(s->C);
     HIP_TRY(hipMemcpyAsync(s->d_ctl, s->h_ctl, sizeof(Ctl), hipMemcpyHostToDevice, s->stream));
     return 0;
 }
@@ -911,8 +1001,9 @@ int do_init(cr_solver* s, uint64_t* base) {
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, s->stream));
-    // level 0 lists are written into buffer 0 (the "next" of cur = 1)
-    LevelArgs a = level_args(s, 1, 0);
+    // init resolves depths 0 and 1; the depth-1 lists go to buffer 1 (the
+    // "next" buffer of cur = 0), which level d = 1 reads
+    LevelArgs a = level_args(s, 0, 1);
     Timer t(s, "init");
     if ((rc = dispatch_k<LaunchInit>(s->k, s, a)) != 0) return rc;
     t.stop();
@@ -956,10 +1047,10 @@ int do_loop(cr_solver* s) {
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, s->stream));
-    uint32_t n0 = static_cast<uint32_t>(s->C), n1 = static_cast<uint32_t>(s->C);
-    int levels = 0;
+    uint32_t n0 = s->h_ctl->cnt0[1], n1 = s->h_ctl->cnt1[1];
+    int levels = 1;  // depth 1 was resolved by initializeCaptures
     int rc = 0;
-    for (uint32_t d = 0; d + 1 < kMaxRound; ++d) {
+    for (uint32_t d = 1; d + 1 < kMaxRound && (n0 || n1); ++d) {
         const int c = static_cast<int>(d & 1);  // level d reads list buffer d & 1
         LevelArgs a = level_args(s, c, d + 1);
         Timer t1(s, "cop_push");  // launched even with no rows: it resets the next counters
