@@ -20,6 +20,7 @@ MAX_N = 255
 MAX_ROUND = 127
 VARIANT_COLUMNS = 1
 VARIANT_MACRO = 2
+TUNING = {"push_filter": 1, "push_blocks": 2, "row_blocks": 3, "levels_per_sync": 4}
 
 STATUS = {
     0: "CR_OK", -1: "CR_ERR_ARG", -2: "CR_ERR_IO", -3: "CR_ERR_EMPTY_GRAPH",
@@ -34,7 +35,7 @@ EXPORTS = (
     "cr_load_graph_file", "cr_load_graph_matrix", "cr_set_variant", "cr_build_aux_graph",
     "cr_initialize_captures", "cr_main_loop", "cr_find_final_result", "cr_save_aux_graph",
     "cr_get_info", "cr_get_stats", "cr_copy_states", "cr_copy_configs", "cr_level_counts",
-    "cr_count_retro_edges", "cr_count_transitions", "cr_set_kernel_timing", "cr_kernel_times", "cr_cache_key",
+    "cr_count_retro_edges", "cr_count_transitions", "cr_set_tuning", "cr_set_kernel_timing", "cr_kernel_times", "cr_cache_key",
 )
 
 
@@ -102,6 +103,7 @@ def lib():
         L.cr_level_counts.argtypes = [vp, u64p]
         L.cr_count_retro_edges.argtypes = [vp, u64p, u64p, u64p]
         L.cr_count_transitions.argtypes = [vp, u64p]
+        L.cr_set_tuning.argtypes = [vp, i32, ctypes.c_int64]
         L.cr_set_kernel_timing.argtypes = [vp, i32]
         L.cr_kernel_times.argtypes = [vp, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int64), i32, ctypes.POINTER(i32)]
@@ -245,6 +247,12 @@ class Solver:
         t = ctypes.c_uint64()
         _check(lib().cr_count_transitions(self._h, ctypes.byref(t)))
         return int(t.value)
+
+    def set_tuning(self, **knobs):
+        """Speed knobs (results never change): push_filter, push_blocks,
+        row_blocks, levels_per_sync."""
+        for name, value in knobs.items():
+            _check(lib().cr_set_tuning(self._h, TUNING[name], int(value)))
 
     def set_kernel_timing(self, enabled=True):
         _check(lib().cr_set_kernel_timing(self._h, 1 if enabled else 0))

@@ -89,11 +89,28 @@ struct GraphView {
 };
 
 struct Ctl {
-    uint32_t cnt0[2];  // list lengths: configurations with new cop-turn states
+    uint32_t level;    // frontier depth d of the level being processed
+    uint32_t done;     // set when no level remains
+    uint32_t cnt0[2];  // per parity: length of the list of configurations with new cop-turn states
     uint32_t cnt1[2];  // ... with new robber-turn states
     unsigned long long lvl[128][2];
     unsigned long long verdict;
     unsigned long long retro[3];
+};
+
+// Row sets are compacted without contended atomics. A producer kernel walks an
+// index space in groups of kGroupWords bitmap words (one workgroup per group,
+// one wavefront per word at a time): it stores one flag word per 64 indices and
+// the group's flag and state counts with plain stores. k_advance scans the
+// group counts into offsets, and k_write_lists turns the flag words into a
+// sorted dense list.
+constexpr int kGroupWords = kWavesPerBlock;  // one flag word per wavefront and group
+constexpr int kWordsPerWave = kGroupWords / kWavesPerBlock;
+
+struct Compact {
+    uint64_t* flags;              // one bit per index of the producer's space
+    uint32_t* gcnt;               // per group: set flags; after the scan: offset
+    unsigned long long* gstates;  // per group: states won
 };
 
 struct LevelArgs {
@@ -102,16 +119,28 @@ struct LevelArgs {
     uint64_t* fr1;
     uint64_t* nx0;
     uint8_t* table;
-    const uint32_t* list0_cur;
-    const uint32_t* list1_cur;
-    uint32_t* list0_nxt;
-    uint32_t* list1_nxt;
+    uint32_t* list0[2];  // per parity: configurations with new cop-turn states (sorted)
+    uint32_t* list1[2];  // ... with new robber-turn states
+    Compact c0;          // cop-turn rows (index space: configurations)
+    Compact c1;          // robber-turn rows (index space: entries of list0, or configurations at init)
+    unsigned long long* gcaps;  // per group of k_init: capture states
     Ctl* ctl;
     const uint64_t* out_mask;  // [n][4] robber moves
-    int32_t cur;
-    uint32_t depth_new;
+    uint32_t nbatch;           // ceil(C / 64)
     int32_t filter;
 };
+
+// Copies the robber out-masks (n rows of W words) into LDS; returns the base.
+template <int W>
+__device__ __forceinline__ const uint64_t* stage_out_masks(uint64_t* lds_om, const uint64_t* out_mask, int n) {
+    for (int i = threadIdx.x; i < n * W; i += blockDim.x) lds_om[i] = out_mask[(i / W) * 4 + (i % W)];
+    __syncthreads();
+    return lds_om;
+}
+
+__device__ __forceinline__ uint32_t wave_id() {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)));
+}
 
 __device__ __forceinline__ uint8_t won_byte(uint32_t d) { return static_cast<uint8_t>(1u | (d << 1)); }
 
@@ -187,9 +216,29 @@ __device__ __forceinline__ bool product_tuple(uint32_t t, const uint32_t* cops, 
     return keep;
 }
 
-__device__ __forceinline__ uint64_t lanemask_lt() {
-    const int lane = threadIdx.x & 63;
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+
+// Block-wide sums of wave-uniform per-wave values; thread 0 gets the totals.
+// Every thread of the block must call it.
+__device__ __forceinline__ void block_sum2(unsigned long long& x, unsigned long long& y) {
+    __shared__ unsigned long long red[2][kWavesPerBlock];
+    __syncthreads();  // previous use of red is finished
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = x;
+        red[1][threadIdx.x >> 6] = y;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        x = y = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) {
+            x += red[0][i];
+            y += red[1][i];
+        }
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -199,139 +248,150 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 // an out-neighbour (that cop steps onto the robber; the others move anywhere),
 // and a robber-turn state iff every move of r lands on a cop of C. This is the
 // push of level 0 (every capture state to all of its predecessors) done per
-// row without touching any other row.
+// row without touching any other row. Lanes = configurations.
 // ---------------------------------------------------------------------------
 template <int K, int W>
 __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint64_t lds_om[CR_MAX_N * W];
     load_image(lds, gv.image, gv.image_bytes);
+    const uint64_t* lom = stage_out_masks<W>(lds_om, a.out_mask, gv.n);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const int lane = threadIdx.x & 63;
-    uint64_t om[W][W];  // out-masks of robber vertices lane + 64 j
-    bool valid[W];
+    const uint32_t wid = wave_id();
+    const int n = gv.n;
+    const uint32_t ngroups = (a.nbatch + kGroupWords - 1) / kGroupWords;
+    for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        unsigned long long caps = 0, cop1 = 0, rob1 = 0, rows0 = 0, rows1 = 0;
+        for (int j = 0; j < kWordsPerWave; ++j) {
+            const uint32_t b = g * kGroupWords + wid * kWordsPerWave + j;
+            if (b >= a.nbatch) break;
+            const uint32_t row = b * 64 + lane;
+            const bool live = row < gv.C;
+            uint64_t cm[W], am[W], r1[W];
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
-        const int r = lane + 64 * j;
-        valid[j] = r < gv.n;
+            for (int q = 0; q < W; ++q) cm[q] = am[q] = r1[q] = 0;
+            if (live) {
+                uint32_t cops[K];
+                unrank<K>(row, gv, binom, cops);
 #pragma unroll
-        for (int q = 0; q < W; ++q) om[j][q] = valid[j] ? a.out_mask[r * 4 + q] : 0ull;
-    }
-    const uint32_t nbatch = (gv.C + 63) / 64;
-    const uint8_t w1b = won_byte(1);
-    unsigned long long caps = 0, cop1 = 0, rob1 = 0;
-    for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
-         b += gridDim.x * kWavesPerBlock) {
-        const uint32_t row = b * 64 + lane;
-        uint64_t cm[W], am[W];
+                for (int i = 0; i < K; ++i) {
 #pragma unroll
-        for (int q = 0; q < W; ++q) cm[q] = am[q] = 0;
-        if (row < gv.C) {
-            uint32_t cops[K];
-            unrank<K>(row, gv, binom, cops);
+                    for (int q = 0; q < W; ++q) {
+                        if ((cops[i] >> 6) == static_cast<uint32_t>(q)) cm[q] |= 1ull << (cops[i] & 63);
+                        am[q] |= lom[cops[i] * W + q];
+                    }
+                }
+            }
+            // robber vertices in a uniform loop (out-masks broadcast from LDS)
 #pragma unroll
-            for (int i = 0; i < K; ++i) {
+            for (int w = 0; w < W; ++w) {
+                const int rend = min(64, n - 64 * w);
+                for (int rb = 0; rb < rend; ++rb) {
+                    const uint64_t* om = lom + (64 * w + rb) * W;
+                    uint64_t bad = (cm[w] >> rb) & 1;
+#pragma unroll
+                    for (int q = 0; q < W; ++q) bad |= om[q] & ~cm[q];
+                    r1[w] |= static_cast<uint64_t>(bad == 0) << rb;
+                }
+            }
+            bool has0 = false, has1 = false;
+            uint32_t lc = 0, l0 = 0, l1 = 0;
+            if (live) {
+                const uint64_t o = static_cast<uint64_t>(row) * W;
+                uint64_t c1[W];
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
-                    if ((cops[i] >> 6) == static_cast<uint32_t>(q)) cm[q] |= 1ull << (cops[i] & 63);
-                    am[q] |= a.out_mask[cops[i] * 4 + q];
+                    c1[q] = am[q] & ~cm[q];
+                    a.won0[o + q] = cm[q] | c1[q];
+                    a.won1[o + q] = cm[q] | r1[q];
+                    a.fr1[o + q] = r1[q];
+                    a.nx0[o + q] = 0;
+                    lc += __popcll(cm[q]);
+                    l0 += __popcll(c1[q]);
+                    l1 += __popcll(r1[q]);
+                    has0 |= c1[q] != 0;
+                    has1 |= r1[q] != 0;
+                }
+                // the row's 2N table bytes: (cap 0x01 | depth-1 0x03 | unmarked 0xFE) per turn
+                uint8_t* trow = a.table + static_cast<uint64_t>(row) * n * 2;
+                auto pos_pair = [&](uint64_t cw, uint64_t c1w, uint64_t r1w, int rb) -> uint32_t {
+                    const uint64_t bit = 1ull << rb;
+                    const bool cap = cw & bit;
+                    const uint32_t b0 = cap ? 0x01u : ((c1w & bit) ? 0x03u : kUnmarked);
+                    const uint32_t b1 = cap ? 0x01u : ((r1w & bit) ? 0x03u : kUnmarked);
+                    return b0 | (b1 << 8);
+                };
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const int rend = min(64, n - 64 * w);
+                    uint8_t* tw = trow + 128 * w;
+                    if ((n & 3) == 0) {  // rows start 8-byte aligned
+                        for (int rb = 0; rb < rend; rb += 4) {
+                            const uint64_t v = static_cast<uint64_t>(pos_pair(cm[w], c1[w], r1[w], rb)) |
+                                               (static_cast<uint64_t>(pos_pair(cm[w], c1[w], r1[w], rb + 1)) << 16) |
+                                               (static_cast<uint64_t>(pos_pair(cm[w], c1[w], r1[w], rb + 2)) << 32) |
+                                               (static_cast<uint64_t>(pos_pair(cm[w], c1[w], r1[w], rb + 3)) << 48);
+                            *reinterpret_cast<uint64_t*>(tw + 2 * rb) = v;
+                        }
+                    } else if ((n & 1) == 0) {  // 4-byte aligned
+                        for (int rb = 0; rb < rend; rb += 2)
+                            *reinterpret_cast<uint32_t*>(tw + 2 * rb) =
+                                pos_pair(cm[w], c1[w], r1[w], rb) | (pos_pair(cm[w], c1[w], r1[w], rb + 1) << 16);
+                    } else {
+                        for (int rb = 0; rb < rend; ++rb)
+                            *reinterpret_cast<uint16_t*>(tw + 2 * rb) =
+                                static_cast<uint16_t>(pos_pair(cm[w], c1[w], r1[w], rb));
+                    }
                 }
             }
-        }
-        uint64_t my_r1[W];
-#pragma unroll
-        for (int q = 0; q < W; ++q) my_r1[q] = 0;
-        const uint32_t rows_here = min(64u, gv.C - b * 64);
-        for (uint32_t i = 0; i < rows_here; ++i) {
-            const uint32_t rr = b * 64 + i;
-            uint64_t c[W], adj[W], r1[W];
-#pragma unroll
-            for (int q = 0; q < W; ++q) {
-                c[q] = __shfl(cm[q], static_cast<int>(i));
-                adj[q] = __shfl(am[q], static_cast<int>(i));
+            const uint64_t f0 = __ballot(has0), f1 = __ballot(has1);
+            if (lane == 0) {
+                a.c0.flags[b] = f0;
+                a.c1.flags[b] = f1;
             }
-            uint16_t* trow = reinterpret_cast<uint16_t*>(a.table + static_cast<uint64_t>(rr) * gv.n * 2);
-#pragma unroll
-            for (int j = 0; j < W; ++j) {
-                const bool cap = (c[j] >> lane) & 1;
-                bool forced = valid[j] && !cap;  // every robber move lands on a cop
-#pragma unroll
-                for (int q = 0; q < W; ++q) forced = forced && ((om[j][q] & ~c[q]) == 0);
-                r1[j] = __ballot(forced);
-                if (valid[j]) {
-                    const uint8_t b0 = cap ? won_byte(0) : (((adj[j] >> lane) & 1) ? w1b : kUnmarked);
-                    const uint8_t b1 = cap ? won_byte(0) : (forced ? w1b : kUnmarked);
-                    trow[lane + 64 * j] = static_cast<uint16_t>(b0 | (b1 << 8));
-                }
-            }
-            if (lane == static_cast<int>(i)) {
-#pragma unroll
-                for (int q = 0; q < W; ++q) my_r1[q] = r1[q];
-            }
+            rows0 += __popcll(f0);
+            rows1 += __popcll(f1);
+            caps += wave_sum(lc);
+            cop1 += wave_sum(l0);
+            rob1 += wave_sum(l1);
         }
-        bool has0 = false, has1 = false;
-        if (row < gv.C) {
-            const uint64_t o = static_cast<uint64_t>(row) * W;
-#pragma unroll
-            for (int q = 0; q < W; ++q) {
-                const uint64_t c1 = am[q] & ~cm[q];
-                a.won0[o + q] = cm[q] | c1;
-                a.won1[o + q] = cm[q] | my_r1[q];
-                a.fr1[o + q] = my_r1[q];
-                a.nx0[o + q] = 0;
-                caps += __popcll(cm[q]);
-                cop1 += __popcll(c1);
-                rob1 += __popcll(my_r1[q]);
-                has0 |= c1 != 0;
-                has1 |= my_r1[q] != 0;
-            }
+        unsigned long long r0 = rows0, r1s = rows1;
+        block_sum2(r0, r1s);
+        block_sum2(caps, cop1);
+        unsigned long long rb1 = rob1, unused = 0;
+        block_sum2(rb1, unused);
+        if (threadIdx.x == 0) {
+            a.c0.gcnt[g] = static_cast<uint32_t>(r0);
+            a.c1.gcnt[g] = static_cast<uint32_t>(r1s);
+            a.gcaps[g] = caps;
+            a.c0.gstates[g] = cop1;
+            a.c1.gstates[g] = rb1;
         }
-        const uint64_t b0 = __ballot(has0), b1 = __ballot(has1);
-        uint32_t p0 = 0, p1 = 0;
-        if (lane == 0) {
-            if (b0) p0 = atomicAdd(&a.ctl->cnt0[a.cur ^ 1], static_cast<uint32_t>(__popcll(b0)));
-            if (b1) p1 = atomicAdd(&a.ctl->cnt1[a.cur ^ 1], static_cast<uint32_t>(__popcll(b1)));
-        }
-        p0 = __shfl(p0, 0);
-        p1 = __shfl(p1, 0);
-        if (has0) a.list0_nxt[p0 + __popcll(b0 & lanemask_lt())] = row;
-        if (has1) a.list1_nxt[p1 + __popcll(b1 & lanemask_lt())] = row;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        caps += __shfl_xor(caps, off);
-        cop1 += __shfl_xor(cop1, off);
-        rob1 += __shfl_xor(rob1, off);
-    }
-    if (lane == 0) {
-        if (caps) {
-            atomicAdd(&a.ctl->lvl[0][0], caps);
-            atomicAdd(&a.ctl->lvl[0][1], caps);
-        }
-        if (cop1) atomicAdd(&a.ctl->lvl[1][0], cop1);
-        if (rob1) atomicAdd(&a.ctl->lvl[1][1], rob1);
     }
 }
 
 // ---------------------------------------------------------------------------
-// cop turn, push: frontier robber-turn rows -> predecessors' nx0
+// cop turn, push: frontier robber-turn rows -> predecessors' nx0. One
+// wavefront per list row: lanes enumerate the team moves back along in-edges,
+// rank them and OR the row's robber mask into nx0 of every predecessor.
 // ---------------------------------------------------------------------------
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     load_image(lds, gv.image, gv.image_bytes);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.ctl->cnt0[a.cur ^ 1] = 0;
-        a.ctl->cnt1[a.cur ^ 1] = 0;
-    }
+    if (a.ctl->done) return;
+    const uint32_t cur = a.ctl->level & 1;
     const uint32_t* rank_tab = reinterpret_cast<const uint32_t*>(lds);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const uint16_t* in_off = reinterpret_cast<const uint16_t*>(lds + gv.off_in_off);
     const uint8_t* in_nbr = lds + gv.off_in_nbr;
     const int lane = threadIdx.x & 63;
     const int W = gv.W;
-    const uint32_t count = a.ctl->cnt1[a.cur];
-    for (uint32_t it = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); it < count;
-         it += gridDim.x * kWavesPerBlock) {
-        const uint32_t row = a.list1_cur[it];
+    const uint32_t count = a.ctl->cnt1[cur];
+    const uint32_t* list = a.list1[cur];
+    for (uint32_t it = blockIdx.x * kWavesPerBlock + wave_id(); it < count; it += gridDim.x * kWavesPerBlock) {
+        const uint32_t row = list[it];
         uint64_t m[4] = {0, 0, 0, 0};
         for (int w = 0; w < W; ++w) m[w] = a.fr1[static_cast<uint64_t>(row) * W + w];
         uint32_t cops[K], deg[K], base[K];
@@ -343,19 +403,32 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
             deg[i] = in_off[cops[i] + 1] - base[i];
             P *= deg[i];
         }
-        for (uint32_t t = lane; t < P; t += 64) {
-            uint32_t tup[K];
-            if (!product_tuple<K>(t, cops, deg, base, in_nbr, tup)) continue;
-            const uint64_t pr = rank_sorted<K>(tup, rank_tab, gv.n, gv.ctot_m1);
+        // two predecessors per lane per iteration keep two filter loads in flight
+        for (uint32_t t = lane; t < P; t += 128) {
+            uint32_t tup[2][K];
+            bool keep[2];
+            uint64_t pr[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const uint32_t tt = t + 64 * u;
+                const bool in = tt < P;  // decode a valid index either way; drop it when out of range
+                keep[u] = product_tuple<K>(in ? tt : t, cops, deg, base, in_nbr, tup[u]) && in;
+                pr[u] = rank_sorted<K>(tup[u], rank_tab, gv.n, gv.ctot_m1);
+            }
             for (int w = 0; w < W; ++w) {
-                uint64_t mw = m[w];
-                if (mw == 0) continue;
+                if (m[w] == 0) continue;
+                uint64_t mw[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) mw[u] = keep[u] ? m[w] : 0ull;
                 if (a.filter) {
-                    mw &= ~a.won0[pr * W + w];
-                    if (mw == 0) continue;
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+                        if (mw[u]) mw[u] &= ~a.won0[pr[u] * W + w];
                 }
-                atomicOr(reinterpret_cast<unsigned long long*>(a.nx0 + pr * W + w),
-                         static_cast<unsigned long long>(mw));
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+                    if (mw[u]) atomicOr(reinterpret_cast<unsigned long long*>(a.nx0 + pr[u] * W + w),
+                                        static_cast<unsigned long long>(mw[u]));
             }
         }
         if (lane < W) a.fr1[static_cast<uint64_t>(row) * W + lane] = 0;
@@ -363,145 +436,271 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
 }
 
 // ---------------------------------------------------------------------------
-// robber turn: rows with new cop-turn states; r won iff out_mask[r] within won0
+// robber turn: for every configuration with new cop-turn states at depth d,
+// robber r is won iff every move of r lands on a won cop-turn state
+// (out_mask[r] within won0); depth = d + 1. Lanes = entries of list0.
 // ---------------------------------------------------------------------------
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs a) {
+    __shared__ uint64_t lds_om[CR_MAX_N * W];
     const int lane = threadIdx.x & 63;
-    uint64_t om[W][W];
-    bool valid[W];
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-        const int r = lane + 64 * j;
-        valid[j] = r < gv.n;
-#pragma unroll
-        for (int q = 0; q < W; ++q) om[j][q] = valid[j] ? a.out_mask[r * 4 + q] : 0ull;
-    }
-    const uint32_t count = a.ctl->cnt0[a.cur];
-    const uint32_t nbatch = (count + 63) / 64;
-    const uint8_t nb = won_byte(a.depth_new);
-    unsigned long long found = 0;
-    for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
-         b += gridDim.x * kWavesPerBlock) {
-        // each lane loads one row of the batch; the row loop then only shuffles
-        const bool live = b * 64 + lane < count;
-        const uint32_t mine = live ? a.list0_cur[b * 64 + lane] : 0u;
-        uint64_t lw0[W], lw1[W], lnew[W];
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            lw0[q] = live ? a.won0[static_cast<uint64_t>(mine) * W + q] : 0ull;
-            lw1[q] = live ? a.won1[static_cast<uint64_t>(mine) * W + q] : 0ull;
-            lnew[q] = 0;
-        }
-        const uint32_t rows_here = min(64u, count - b * 64);
-        for (uint32_t i = 0; i < rows_here; ++i) {
-            const uint32_t row = __shfl(mine, static_cast<int>(i));
+    if (a.ctl->done) return;
+    const uint64_t* lom = stage_out_masks<W>(lds_om, a.out_mask, gv.n);
+    const uint32_t level = a.ctl->level, cur = level & 1;
+    const uint32_t count = a.ctl->cnt0[cur];
+    const uint32_t* list = a.list0[cur];
+    const uint32_t nwords = (count + 63) / 64;
+    const uint32_t ngroups = (nwords + kGroupWords - 1) / kGroupWords;
+    const uint32_t wid = wave_id();
+    const uint8_t nb = won_byte(level + 1);
+    const int n = gv.n;
+    for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        unsigned long long rows = 0, found = 0;
+        for (int j = 0; j < kWordsPerWave; ++j) {
+            const uint32_t b = g * kGroupWords + wid * kWordsPerWave + j;
+            if (b >= nwords) break;
+            const uint32_t i = b * 64 + lane;
+            const bool live = i < count;
+            const uint32_t row = live ? list[i] : 0u;
+            const uint64_t o = static_cast<uint64_t>(row) * W;
             uint64_t w0[W], w1[W], nw[W];
 #pragma unroll
             for (int q = 0; q < W; ++q) {
-                w0[q] = __shfl(lw0[q], static_cast<int>(i));
-                w1[q] = __shfl(lw1[q], static_cast<int>(i));
+                w0[q] = live ? a.won0[o + q] : 0ull;
+                w1[q] = live ? a.won1[o + q] : ~0ull;
+                nw[q] = 0;
             }
 #pragma unroll
-            for (int j = 0; j < W; ++j) {
-                bool ok = valid[j] && !((w1[j] >> lane) & 1);
+            for (int w = 0; w < W; ++w) {
+                const int rend = min(64, n - 64 * w);
+                for (int rb = 0; rb < rend; ++rb) {
+                    const uint64_t* om = lom + (64 * w + rb) * W;
+                    uint64_t bad = (w1[w] >> rb) & 1;
 #pragma unroll
-                for (int q = 0; q < W; ++q) ok = ok && ((om[j][q] & ~w0[q]) == 0);
-                nw[j] = __ballot(ok);
-                if (ok) a.table[(static_cast<uint64_t>(row) * gv.n + lane + 64 * j) * 2 + 1] = nb;
+                    for (int q = 0; q < W; ++q) bad |= om[q] & ~w0[q];
+                    nw[w] |= static_cast<uint64_t>(bad == 0) << rb;
+                }
             }
-            if (lane == static_cast<int>(i)) {
-#pragma unroll
-                for (int q = 0; q < W; ++q) lnew[q] = nw[q];
-            }
-        }
-        bool append = false;
-        if (live) {
-            const uint64_t o = static_cast<uint64_t>(mine) * W;
             uint32_t pop = 0;
 #pragma unroll
-            for (int q = 0; q < W; ++q) pop += __popcll(lnew[q]);
-            if (pop) {
+            for (int q = 0; q < W; ++q) pop += __popcll(nw[q]);
+            const bool append = pop != 0;
+            if (append) {
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
-                    a.won1[o + q] = lw1[q] | lnew[q];
-                    a.fr1[o + q] = lnew[q];
+                    a.won1[o + q] = w1[q] | nw[q];
+                    a.fr1[o + q] = nw[q];
+                    uint64_t bits = nw[q];
+                    while (bits) {
+                        const int r = q * 64 + __builtin_ctzll(bits);
+                        bits &= bits - 1;
+                        a.table[(static_cast<uint64_t>(row) * n + r) * 2 + 1] = nb;
+                    }
                 }
-                append = true;
-                found += pop;
             }
+            const uint64_t f = __ballot(append);
+            if (lane == 0) a.c1.flags[b] = f;
+            rows += __popcll(f);
+            found += wave_sum(pop);
         }
-        const uint64_t bal = __ballot(append);
-        if (bal) {
-            uint32_t pos = 0;
-            if (lane == 0) pos = atomicAdd(&a.ctl->cnt1[a.cur ^ 1], static_cast<uint32_t>(__popcll(bal)));
-            pos = __shfl(pos, 0);
-            if (append) a.list1_nxt[pos + __popcll(bal & lanemask_lt())] = mine;
+        block_sum2(rows, found);
+        if (threadIdx.x == 0) {
+            a.c1.gcnt[g] = static_cast<uint32_t>(rows);
+            a.c1.gstates[g] = found;
         }
     }
-    for (int off = 32; off > 0; off >>= 1) found += __shfl_xor(found, off);
-    if (lane == 0 && found) atomicAdd(&a.ctl->lvl[a.depth_new][1], found);
 }
 
 // ---------------------------------------------------------------------------
-// cop turn, finalise: nx0 & ~won0 -> new cop-turn states (full scan of nx0)
+// cop turn, finalise: nx0 & ~won0 -> new cop-turn states (full scan of nx0,
+// lanes = configurations)
 // ---------------------------------------------------------------------------
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs a) {
     const int lane = threadIdx.x & 63;
-    const uint32_t nbatch = (gv.C + 63) / 64;
-    const uint8_t nb = won_byte(a.depth_new);
-    unsigned long long found = 0;
-    for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
-         b += gridDim.x * kWavesPerBlock) {
-        const uint32_t row = b * 64 + lane;
-        const uint64_t o = static_cast<uint64_t>(row) * W;
-        uint64_t x[W];
-        bool any = false;
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            x[q] = row < gv.C ? a.nx0[o + q] : 0ull;
-            any |= x[q] != 0;
-        }
-        uint64_t nwl[W];
-        bool append = false;
-        if (any) {  // this lane's row has candidates: settle them
-            uint32_t pop = 0;
+    if (a.ctl->done) return;
+    const uint32_t level = a.ctl->level;
+    const uint8_t nb = won_byte(level + 1);
+    const uint32_t ngroups = (a.nbatch + kGroupWords - 1) / kGroupWords;
+    const uint32_t wid = wave_id();
+    for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        unsigned long long rows = 0, found = 0;
+        for (int j = 0; j < kWordsPerWave; ++j) {
+            const uint32_t b = g * kGroupWords + wid * kWordsPerWave + j;
+            if (b >= a.nbatch) break;
+            const uint32_t row = b * 64 + lane;
+            const uint64_t o = static_cast<uint64_t>(row) * W;
+            uint64_t x[W];
+            bool any = false;
 #pragma unroll
             for (int q = 0; q < W; ++q) {
-                const uint64_t wv = a.won0[o + q];
-                nwl[q] = x[q] & ~wv;
-                pop += __popcll(nwl[q]);
-                a.nx0[o + q] = 0;
-                if (nwl[q]) a.won0[o + q] = wv | nwl[q];
+                x[q] = row < gv.C ? a.nx0[o + q] : 0ull;
+                any |= x[q] != 0;
             }
-            append = pop != 0;
-            found += pop;
-        } else {
+            uint32_t pop = 0;
+            if (any) {  // this lane's row has candidates: settle them
 #pragma unroll
-            for (int q = 0; q < W; ++q) nwl[q] = 0;
-        }
-        uint64_t bal = __ballot(append);
-        const uint64_t ab = bal;
-        while (bal) {  // table bytes, one row at a time with lanes = robber vertices
-            const int src = __builtin_ctzll(bal);
-            bal &= bal - 1;
-            const uint32_t rr = b * 64 + src;
-#pragma unroll
-            for (int j = 0; j < W; ++j) {
-                const uint64_t nw = __shfl(nwl[j], src);
-                if ((nw >> lane) & 1) a.table[(static_cast<uint64_t>(rr) * gv.n + lane + 64 * j) * 2] = nb;
+                for (int q = 0; q < W; ++q) {
+                    const uint64_t wv = a.won0[o + q];
+                    const uint64_t nw = x[q] & ~wv;
+                    pop += __popcll(nw);
+                    a.nx0[o + q] = 0;
+                    if (nw) a.won0[o + q] = wv | nw;
+                    uint64_t bits = nw;
+                    while (bits) {
+                        const int r = q * 64 + __builtin_ctzll(bits);
+                        bits &= bits - 1;
+                        a.table[(static_cast<uint64_t>(row) * gv.n + r) * 2] = nb;
+                    }
+                }
             }
+            const uint64_t f = __ballot(pop != 0);
+            if (lane == 0) a.c0.flags[b] = f;
+            rows += __popcll(f);
+            found += wave_sum(pop);
         }
-        if (ab) {
-            uint32_t pos = 0;
-            if (lane == 0) pos = atomicAdd(&a.ctl->cnt0[a.cur ^ 1], static_cast<uint32_t>(__popcll(ab)));
-            pos = __shfl(pos, 0);
-            if (append) a.list0_nxt[pos + __popcll(ab & lanemask_lt())] = row;
+        block_sum2(rows, found);
+        if (threadIdx.x == 0) {
+            a.c0.gcnt[g] = static_cast<uint32_t>(rows);
+            a.c0.gstates[g] = found;
         }
     }
-    for (int off = 32; off > 0; off >>= 1) found += __shfl_xor(found, off);
-    if (lane == 0 && found) atomicAdd(&a.ctl->lvl[a.depth_new][0], found);
+}
+
+// Block-wide exclusive scan of gcnt[0, ng) in place; returns the total and the
+// sum of gstates[0, ng) to thread 0 (one workgroup of kScanThreads threads).
+constexpr int kScanThreads = 1024;
+
+__device__ void scan_groups(uint32_t* gcnt, const unsigned long long* gstates, uint32_t ng,
+                            uint32_t& total, unsigned long long& states) {
+    __shared__ uint32_t part[kScanThreads];
+    __shared__ unsigned long long spart[kScanThreads / 64];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (ng + kScanThreads - 1) / kScanThreads;
+    const uint32_t lo = min(ng, t * per), hi = min(ng, lo + per);
+    uint32_t sum = 0;
+    unsigned long long st = 0;
+    for (uint32_t i = lo; i < hi; ++i) {
+        sum += gcnt[i];
+        st += gstates[i];
+    }
+    part[t] = sum;
+    st = wave_sum(st);
+    if ((t & 63) == 0) spart[t >> 6] = st;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {  // Hillis-Steele inclusive scan
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;  // exclusive prefix of this thread's chunk
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t c = gcnt[i];
+        gcnt[i] = run;
+        run += c;
+    }
+    if (t == 0) {
+        total = part[kScanThreads - 1];
+        states = 0;
+        for (int i = 0; i < kScanThreads / 64; ++i) states += spart[i];
+    }
+    __syncthreads();
+}
+
+// Ends a level: scans the group counts of both producers into list offsets,
+// adds the level's state counts to the histogram and moves to depth level+1;
+// stops when no row holds a new state or when the next depth would have to
+// reach the 127 cap (levels run d = 1..125).
+__global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
+    Ctl* ctl = a.ctl;
+    if (ctl->done) return;
+    const uint32_t d = ctl->level, cur = d & 1, nxt = cur ^ 1;
+    const uint32_t ng0 = (a.nbatch + kGroupWords - 1) / kGroupWords;
+    const uint32_t nw1 = (ctl->cnt0[cur] + 63) / 64;
+    const uint32_t ng1 = (nw1 + kGroupWords - 1) / kGroupWords;
+    uint32_t t0 = 0, t1 = 0;
+    unsigned long long s0 = 0, s1 = 0;
+    scan_groups(a.c0.gcnt, a.c0.gstates, ng0, t0, s0);
+    scan_groups(a.c1.gcnt, a.c1.gstates, ng1, t1, s1);
+    if (threadIdx.x == 0) {
+        ctl->lvl[d + 1][0] = s0;
+        ctl->lvl[d + 1][1] = s1;
+        ctl->cnt0[nxt] = t0;
+        ctl->cnt1[nxt] = t1;
+        ctl->level = d + 1;
+        if ((t0 == 0 && t1 == 0) || d + 1 >= kMaxRound - 1) ctl->done = 1;
+    }
+}
+
+// After k_init: offsets for the depth-1 lists (parity 1) and the histogram
+// entries of depths 0 and 1.
+__global__ __launch_bounds__(kScanThreads) void k_settle_init(LevelArgs a) {
+    Ctl* ctl = a.ctl;
+    const uint32_t ng = (a.nbatch + kGroupWords - 1) / kGroupWords;
+    uint32_t t0 = 0, t1 = 0, tc = 0;
+    unsigned long long s0 = 0, s1 = 0, caps = 0;
+    scan_groups(a.c0.gcnt, a.c0.gstates, ng, t0, s0);
+    scan_groups(a.c1.gcnt, a.c1.gstates, ng, t1, s1);
+    // capture counts: sum only (the scan result is not used)
+    __shared__ unsigned long long cred[kScanThreads / 64];
+    unsigned long long c = 0;
+    for (uint32_t i = threadIdx.x; i < ng; i += blockDim.x) c += a.gcaps[i];
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) cred[threadIdx.x >> 6] = c;
+    __syncthreads();
+    (void)tc;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < kScanThreads / 64; ++i) caps += cred[i];
+        ctl->lvl[0][0] = caps;
+        ctl->lvl[0][1] = caps;
+        ctl->lvl[1][0] = s0;
+        ctl->lvl[1][1] = s1;
+        ctl->cnt0[1] = t0;
+        ctl->cnt1[1] = t1;
+        ctl->level = 1;
+        ctl->done = (t0 == 0 && t1 == 0) ? 1u : 0u;
+    }
+}
+
+// Dense sorted lists from the flag words: list0 of the next parity from the
+// finalize flags (values = configurations) and list1 from the robber flags
+// (values = the list0 entries they flag). One thread per flag word; the
+// words of a group are adjacent lanes, so a segmented scan over kGroupWords
+// lanes gives each word's offset inside its group. init_mode serves k_init
+// (both spaces are configurations, lists go to parity 1).
+__global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t init_mode) {
+    const Ctl* ctl = a.ctl;
+    if (ctl->done) return;  // no further level reads the lists
+    // after k_advance, level already names the next depth: its parity holds the new lists
+    const uint32_t nxt = init_mode ? 1u : (ctl->level & 1);
+    const uint32_t prev = nxt ^ 1;
+    const uint32_t nw0 = a.nbatch;
+    const uint32_t nw1 = init_mode ? a.nbatch : (ctl->cnt0[prev] + 63) / 64;
+    const uint32_t span0 = (nw0 + kGroupWords - 1) / kGroupWords * kGroupWords;  // space 0 words, padded
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t - lane < span0 + nw1; t += gridDim.x * blockDim.x) {
+        const bool space1 = t >= span0;
+        const uint32_t word = space1 ? t - span0 : t;
+        const Compact& c = space1 ? a.c1 : a.c0;
+        const uint32_t nw = space1 ? nw1 : nw0;
+        const uint64_t bits = word < nw ? c.flags[word] : 0ull;
+        const uint32_t pc = __popcll(bits);
+        uint32_t incl = pc;
+        for (int off = 1; off < kGroupWords; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off, kGroupWords);
+            if (static_cast<int>(word % kGroupWords) >= off) incl += v;
+        }
+        if (bits == 0) continue;
+        uint32_t pos = c.gcnt[word / kGroupWords] + incl - pc;
+        uint32_t* dst = space1 ? a.list1[nxt] : a.list0[nxt];
+        uint64_t rest = bits;
+        while (rest) {
+            const uint32_t idx = word * 64 + static_cast<uint32_t>(__builtin_ctzll(rest));
+            rest &= rest - 1;
+            dst[pos++] = (space1 && !init_mode) ? a.list0[prev][idx] : idx;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -658,14 +857,32 @@ struct cr_solver {
     uint64_t* d_won1 = nullptr;
     uint64_t* d_fr1 = nullptr;
     uint64_t* d_nx0 = nullptr;
-    uint32_t* d_list[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [turn][buffer]
+    uint32_t* d_list[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [turn][parity] dense row lists
+    uint64_t* d_flags[2] = {nullptr, nullptr};                          // producer flag words per turn
+    uint32_t* d_gcnt[2] = {nullptr, nullptr};                           // per-group counts / offsets
+    unsigned long long* d_gstates[2] = {nullptr, nullptr};              // per-group state counts
+    unsigned long long* d_gcaps = nullptr;
     Ctl* d_ctl = nullptr;
     uint8_t* d_image = nullptr;
     uint64_t* d_outmask = nullptr;
     Ctl* h_ctl = nullptr;  // pinned
     uint64_t device_bytes = 0;
     GraphView gv{};
+    // tuning knobs (cr_set_tuning)
     int32_t filter = 1;
+    uint32_t push_blocks = 2048;
+    uint32_t row_blocks = 2048;
+    int32_t levels_per_sync = 8;
+
+    uint32_t nbatch() const { return static_cast<uint32_t>((C + 63) / 64); }
+    uint64_t ngroups() const { return (nbatch() + kGroupWords - 1) / kGroupWords; }
+    // grid of the row kernels (one workgroup per group of flag words, grid-stride)
+    uint32_t row_grid() const {
+        return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(ngroups(), row_blocks)));
+    }
+    uint32_t writer_grid() const {
+        return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((2 * nbatch() + 255) / 256 + 1, 4096)));
+    }
     // results
     std::vector<uint64_t> pass_counts;
     cr_stats stats{};
@@ -683,6 +900,16 @@ struct cr_solver {
         hipFree(d_nx0);
         for (auto& t : d_list)
             for (auto& b : t) { hipFree(b); b = nullptr; }
+        for (int i = 0; i < 2; ++i) {
+            hipFree(d_flags[i]);
+            hipFree(d_gcnt[i]);
+            hipFree(d_gstates[i]);
+            d_flags[i] = nullptr;
+            d_gcnt[i] = nullptr;
+            d_gstates[i] = nullptr;
+        }
+        hipFree(d_gcaps);
+        d_gcaps = nullptr;
         hipFree(d_ctl);
         hipFree(d_image);
         hipFree(d_outmask);
@@ -781,8 +1008,11 @@ int raise_lds_limit(const void* fn, uint32_t bytes) {
 template <int K, int W>
 int launch_init(cr_solver* s, LevelArgs a) {
     if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_init<K, W>), s->gv.image_bytes)) return rc;
-    hipLaunchKernelGGL((k_init<K, W>), dim3(grid_for((s->C + 63) / 64, 2048)), dim3(kBlock), s->gv.image_bytes,
-                       s->stream, s->gv, a);
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(s->ngroups(), 2048));
+    hipLaunchKernelGGL((k_init<K, W>), dim3(std::max<uint32_t>(grid, 1)), dim3(kBlock), s->gv.image_bytes, s->stream,
+                       s->gv, a);
+    hipLaunchKernelGGL(k_settle_init, dim3(1), dim3(kScanThreads), 0, s->stream, a);
+    hipLaunchKernelGGL(k_write_lists, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, a, 1);
     return 0;
 }
 
@@ -799,11 +1029,13 @@ struct LaunchInit {
     }
 };
 
+// Level kernels read their level and list lengths from the device, so their
+// grids are sized by the tuning knobs, not by host-side counts.
 template <int K>
 struct LaunchPush {
-    static int run(cr_solver* s, LevelArgs a, uint32_t rows) {
+    static int run(cr_solver* s, LevelArgs a) {
         if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_cop_push<K>), s->gv.image_bytes)) return rc;
-        hipLaunchKernelGGL(k_cop_push<K>, dim3(grid_for(rows, 2048)), dim3(kBlock), s->gv.image_bytes, s->stream,
+        hipLaunchKernelGGL(k_cop_push<K>, dim3(s->push_blocks), dim3(kBlock), s->gv.image_bytes, s->stream,
                            s->gv, a);
         return 0;
     }
@@ -811,9 +1043,8 @@ struct LaunchPush {
 
 template <int W>
 struct LaunchRobber {
-    static int run(cr_solver* s, LevelArgs a, uint32_t rows) {
-        hipLaunchKernelGGL(k_robber_step<W>, dim3(grid_for((rows + 63) / 64)), dim3(kBlock), 0, s->stream,
-                           s->gv, a);
+    static int run(cr_solver* s, LevelArgs a) {
+        hipLaunchKernelGGL(k_robber_step<W>, dim3(s->row_grid()), dim3(kBlock), 0, s->stream, s->gv, a);
         return 0;
     }
 };
@@ -821,8 +1052,7 @@ struct LaunchRobber {
 template <int W>
 struct LaunchFinalize {
     static int run(cr_solver* s, LevelArgs a) {
-        hipLaunchKernelGGL(k_cop_finalize<W>, dim3(grid_for((s->C + 63) / 64)), dim3(kBlock), 0, s->stream,
-                           s->gv, a);
+        hipLaunchKernelGGL(k_cop_finalize<W>, dim3(s->row_grid()), dim3(kBlock), 0, s->stream, s->gv, a);
         return 0;
     }
 };
@@ -850,21 +1080,23 @@ struct LaunchRetro {
     }
 };
 
-LevelArgs level_args(cr_solver* s, int cur, uint32_t depth_new) {
+LevelArgs level_args(cr_solver* s) {
     LevelArgs a{};
     a.won0 = s->d_won0;
     a.won1 = s->d_won1;
     a.fr1 = s->d_fr1;
     a.nx0 = s->d_nx0;
     a.table = s->d_table;
-    a.list0_cur = s->d_list[0][cur];
-    a.list1_cur = s->d_list[1][cur];
-    a.list0_nxt = s->d_list[0][cur ^ 1];
-    a.list1_nxt = s->d_list[1][cur ^ 1];
+    for (int b = 0; b < 2; ++b) {
+        a.list0[b] = s->d_list[0][b];
+        a.list1[b] = s->d_list[1][b];
+    }
+    a.c0 = Compact{s->d_flags[0], s->d_gcnt[0], s->d_gstates[0]};
+    a.c1 = Compact{s->d_flags[1], s->d_gcnt[1], s->d_gstates[1]};
+    a.gcaps = s->d_gcaps;
+    a.nbatch = s->nbatch();
     a.ctl = s->d_ctl;
     a.out_mask = s->d_outmask;
-    a.cur = cur;
-    a.depth_new = depth_new;
     a.filter = s->filter;
     return a;
 }
@@ -965,7 +1197,7 @@ int do_build(cr_solver* s) {
     s->S = s->C * static_cast<uint64_t>(n) * s->columns;
     s->W = (n + 63) / 64;
     const uint64_t plane = s->C * s->W;
-    const uint64_t need = s->S + 4 * plane * 8 + 4 * s->C * 4 + (1 << 20);
+    const uint64_t need = s->S + 4 * plane * 8 + 4 * s->C * 4 + 2 * s->C / 4 + (4 << 20);
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     if (need > freeb) return fail(CR_ERR_TOO_LARGE, "state space needs %llu bytes; %zu free", static_cast<unsigned long long>(need), freeb);
@@ -978,6 +1210,13 @@ int do_build(cr_solver* s) {
     for (auto& t : s->d_list)
         for (auto& b : t)
             if (!(b = dev_alloc<uint32_t>(s->C, s->device_bytes, rc))) return rc;
+    const uint64_t ngroups = (s->nbatch() + kGroupWords - 1) / kGroupWords;
+    for (int i = 0; i < 2; ++i) {
+        if (!(s->d_flags[i] = dev_alloc<uint64_t>(s->nbatch(), s->device_bytes, rc))) return rc;
+        if (!(s->d_gcnt[i] = dev_alloc<uint32_t>(ngroups, s->device_bytes, rc))) return rc;
+        if (!(s->d_gstates[i] = dev_alloc<unsigned long long>(ngroups, s->device_bytes, rc))) return rc;
+    }
+    if (!(s->d_gcaps = dev_alloc<unsigned long long>(ngroups, s->device_bytes, rc))) return rc;
     if (!(s->d_ctl = dev_alloc<Ctl>(1, s->device_bytes, rc))) return rc;
     if (!s->h_ctl) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_ctl), sizeof(Ctl), hipHostMallocDefault));
     if ((rc = upload_graph(s)) != 0) return rc;
@@ -1001,9 +1240,9 @@ int do_init(cr_solver* s, uint64_t* base) {
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, s->stream));
-    // init resolves depths 0 and 1; the depth-1 lists go to buffer 1 (the
-    // "next" buffer of cur = 0), which level d = 1 reads
-    LevelArgs a = level_args(s, 0, 1);
+    // init resolves depths 0 and 1; the depth-1 lists go to parity 1, which
+    // level d = 1 reads
+    LevelArgs a = level_args(s);
     Timer t(s, "init");
     if ((rc = dispatch_k<LaunchInit>(s->k, s, a)) != 0) return rc;
     t.stop();
@@ -1047,31 +1286,33 @@ int do_loop(cr_solver* s) {
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, s->stream));
-    uint32_t n0 = s->h_ctl->cnt0[1], n1 = s->h_ctl->cnt1[1];
-    int levels = 1;  // depth 1 was resolved by initializeCaptures
+    LevelArgs a = level_args(s);
     int rc = 0;
-    for (uint32_t d = 1; d + 1 < kMaxRound && (n0 || n1); ++d) {
-        const int c = static_cast<int>(d & 1);  // level d reads list buffer d & 1
-        LevelArgs a = level_args(s, c, d + 1);
-        Timer t1(s, "cop_push");  // launched even with no rows: it resets the next counters
-        if ((rc = dispatch_k<LaunchPush>(s->k, s, a, n1)) != 0) return rc;
-        t1.stop();
-        Timer t2(s, "robber_step");
-        if (n0 && (rc = dispatch_w<LaunchRobber>(s->W, s, a, n0)) != 0) return rc;
-        t2.stop();
-        Timer t3(s, "cop_finalize");
-        if ((rc = dispatch_w<LaunchFinalize>(s->W, s, a)) != 0) return rc;
-        t3.stop();
+    bool done = s->h_ctl->done != 0;
+    // Levels are enqueued in chunks without host synchronisation: every level
+    // kernel reads its depth and list lengths from the control block, and
+    // k_advance ends the level (or flags completion, after which the queued
+    // kernels return at once).
+    while (!done) {
+        std::vector<std::unique_ptr<Timer>> timers;
+        for (int l = 0; l < s->levels_per_sync; ++l) {
+            timers.emplace_back(new Timer(s, "cop_push"));
+            if ((rc = dispatch_k<LaunchPush>(s->k, s, a)) != 0) return rc;
+            timers.back()->stop();
+            timers.emplace_back(new Timer(s, "robber_step"));
+            if ((rc = dispatch_w<LaunchRobber>(s->W, s, a)) != 0) return rc;
+            timers.back()->stop();
+            timers.emplace_back(new Timer(s, "cop_finalize"));
+            if ((rc = dispatch_w<LaunchFinalize>(s->W, s, a)) != 0) return rc;
+            timers.back()->stop();
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(kScanThreads), 0, s->stream, a);
+            hipLaunchKernelGGL(k_write_lists, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, a, 0);
+        }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl, offsetof(Ctl, lvl), hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
-        t1.collect();
-        t2.collect();
-        t3.collect();
-        ++levels;
-        n0 = s->h_ctl->cnt0[c ^ 1];
-        n1 = s->h_ctl->cnt1[c ^ 1];
-        if (n0 == 0 && n1 == 0) break;
+        for (auto& t : timers) t->collect();
+        done = s->h_ctl->done != 0;
     }
     HIP_TRY(hipEventRecord(e1, s->stream));
     HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
@@ -1081,7 +1322,7 @@ int do_loop(cr_solver* s) {
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     s->stats.loop_ms = ms;
-    s->stats.n_levels = levels;
+    s->stats.n_levels = static_cast<int32_t>(s->h_ctl->level);
     uint64_t won = 0;
     for (int d = 0; d < 127; ++d) won += s->h_ctl->lvl[d][0] + s->h_ctl->lvl[d][1];
     s->stats.won_states = won;
@@ -1103,7 +1344,7 @@ int count_moves(cr_solver* s, int32_t mode, unsigned long long* r) {
         return fail(CR_ERR_UNSUPPORTED, "move count needs %llu bytes of LDS", static_cast<unsigned long long>(lds));
     unsigned long long zero[3] = {0, 0, 0};
     HIP_TRY(hipMemcpyAsync(s->d_ctl->retro, zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
-    LevelArgs a = level_args(s, 0, 0);
+    LevelArgs a = level_args(s);
     if (int rc = dispatch_k<LaunchRetro>(s->k, s, a, slots, mode)) return rc;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(r, s->d_ctl->retro, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
@@ -1213,7 +1454,7 @@ int cr_find_final_result(cr_solver* s, cr_verdict* out) {
     HIP_TRY(hipSetDevice(s->device));
     const unsigned long long init = ~0ull;
     HIP_TRY(hipMemcpyAsync(&s->d_ctl->verdict, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
-    LevelArgs a = level_args(s, 0, 0);
+    LevelArgs a = level_args(s);
     if (int rc = dispatch_w<LaunchVerdict>(s->W, s, a)) return rc;
     HIP_TRY(hipGetLastError());
     unsigned long long v = 0;
@@ -1319,6 +1560,26 @@ int cr_count_transitions(cr_solver* s, uint64_t* total) {
     if (int rc = count_moves(s, 1, r)) return rc;
     *total = r[0];
     return 0;
+}
+
+int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
+    if (int rc = check_solver(s)) return rc;
+    switch (key) {
+        case CR_TUNE_PUSH_FILTER: s->filter = value != 0; return 0;
+        case CR_TUNE_PUSH_BLOCKS:
+            if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "push blocks %lld", static_cast<long long>(value));
+            s->push_blocks = static_cast<uint32_t>(value);
+            return 0;
+        case CR_TUNE_ROW_BLOCKS:
+            if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "row blocks %lld", static_cast<long long>(value));
+            s->row_blocks = static_cast<uint32_t>(value);
+            return 0;
+        case CR_TUNE_LEVELS_PER_SYNC:
+            if (value < 1 || value > 128) return fail(CR_ERR_ARG, "levels per sync %lld", static_cast<long long>(value));
+            s->levels_per_sync = static_cast<int32_t>(value);
+            return 0;
+        default: return fail(CR_ERR_ARG, "unknown tuning key %d", key);
+    }
 }
 
 int cr_set_kernel_timing(cr_solver* s, int32_t enabled) {

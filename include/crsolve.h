@@ -145,6 +145,15 @@ int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_
 /* Total team moves sum over C of |T(C)|: the "Total edge pointers" the
  * reference prints after AuxGraph::createTransitions (AuxGraph.h:274). */
 int cr_count_transitions(cr_solver* s, uint64_t* total);
+/* Tuning knobs (no counterpart in the reference; they change speed, never
+ * results). */
+enum cr_tuning {
+    CR_TUNE_PUSH_FILTER = 1,    /* 1: cop push skips targets already won (default 1) */
+    CR_TUNE_PUSH_BLOCKS = 2,    /* workgroups of the cop-push kernel (default 2048) */
+    CR_TUNE_ROW_BLOCKS = 3,     /* workgroups of the row kernels (default 2048) */
+    CR_TUNE_LEVELS_PER_SYNC = 4 /* depth levels enqueued per host synchronisation (default 4) */
+};
+int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:
  * name_out receives a '\0'-separated list, ms/launches one entry per name. */
 int cr_set_kernel_timing(cr_solver* s, int32_t enabled);
