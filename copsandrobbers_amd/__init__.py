@@ -20,7 +20,8 @@ MAX_N = 255
 MAX_ROUND = 127
 VARIANT_COLUMNS = 1
 VARIANT_MACRO = 2
-TUNING = {"push_filter": 1, "push_blocks": 2, "row_blocks": 3, "levels_per_sync": 4}
+TUNING = {"push_filter": 1, "push_blocks": 2, "row_blocks": 3, "levels_per_sync": 4, "pred_csr": 5,
+          "pull_scale": 6}
 
 STATUS = {
     0: "CR_OK", -1: "CR_ERR_ARG", -2: "CR_ERR_IO", -3: "CR_ERR_EMPTY_GRAPH",
@@ -60,7 +61,8 @@ class Verdict(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("base_captures", ctypes.c_uint64), ("won_states", ctypes.c_uint64),
                 ("n_passes", ctypes.c_int32), ("n_levels", ctypes.c_int32),
-                ("init_ms", ctypes.c_double), ("loop_ms", ctypes.c_double)]
+                ("init_ms", ctypes.c_double), ("loop_ms", ctypes.c_double),
+                ("pred_entries", ctypes.c_uint64), ("build_ms", ctypes.c_double)]
 
 
 _lib = None

@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -93,6 +94,7 @@ struct Ctl {
     uint32_t done;     // set when no level remains
     uint32_t cnt0[2];  // per parity: length of the list of configurations with new cop-turn states
     uint32_t cnt1[2];  // ... with new robber-turn states
+    uint32_t pull;     // the level's cop turn gathers (pull) instead of pushing
     unsigned long long lvl[128][2];
     unsigned long long verdict;
     unsigned long long retro[3];
@@ -128,7 +130,29 @@ struct LevelArgs {
     const uint64_t* out_mask;  // [n][4] robber moves
     uint32_t nbatch;           // ceil(C / 64)
     int32_t filter;
+    uint32_t pull_rows;        // a level pulls when its robber frontier has more rows than this
 };
+
+// Work items [0, n) in 8 contiguous segments, one per XCD (workgroups are
+// dispatched round-robin over the XCDs, so workgroup b runs on XCD b % 8 and
+// the grid is a multiple of 8): neighbouring configurations, whose moves
+// reach overlapping rows, are processed by one XCD and share its L2.
+constexpr uint32_t kXcds = 8;
+struct XcdSplit {
+    uint32_t end, first, stride;
+};
+__device__ __forceinline__ uint32_t wave_id();
+__device__ __forceinline__ XcdSplit xcd_split(uint32_t n) {
+    const uint32_t x = blockIdx.x % kXcds, bpx = gridDim.x / kXcds, lb = blockIdx.x / kXcds;
+    const uint32_t begin = static_cast<uint32_t>(static_cast<uint64_t>(n) * x / kXcds);
+    const uint32_t end = static_cast<uint32_t>(static_cast<uint64_t>(n) * (x + 1) / kXcds);
+    return XcdSplit{end, begin + lb * kWavesPerBlock + wave_id(), bpx * kWavesPerBlock};
+}
+
+// Whether the level starting with `frontier_rows` robber-frontier rows pulls.
+__device__ __forceinline__ uint32_t choose_pull(uint32_t frontier_rows, uint32_t threshold) {
+    return frontier_rows > threshold ? 1u : 0u;
+}
 
 // Copies the robber out-masks (n rows of W words) into LDS; returns the base.
 template <int W>
@@ -379,8 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
+    if (a.ctl->done || a.ctl->pull) return;  // uniform: before any barrier
     load_image(lds, gv.image, gv.image_bytes);
-    if (a.ctl->done) return;
     const uint32_t cur = a.ctl->level & 1;
     const uint32_t* rank_tab = reinterpret_cast<const uint32_t*>(lds);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
@@ -390,7 +414,8 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
     const int W = gv.W;
     const uint32_t count = a.ctl->cnt1[cur];
     const uint32_t* list = a.list1[cur];
-    for (uint32_t it = blockIdx.x * kWavesPerBlock + wave_id(); it < count; it += gridDim.x * kWavesPerBlock) {
+    const XcdSplit sp = xcd_split(count);
+    for (uint32_t it = sp.first; it < sp.end; it += sp.stride) {
         const uint32_t row = list[it];
         uint64_t m[4] = {0, 0, 0, 0};
         for (int w = 0; w < W; ++w) m[w] = a.fr1[static_cast<uint64_t>(row) * W + w];
@@ -433,6 +458,114 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
         }
         if (lane < W) a.fr1[static_cast<uint64_t>(row) * W + lane] = 0;
     }
+}
+
+// ---------------------------------------------------------------------------
+// cop turn, factorised pull (dense levels). The cop-turn candidates of C are
+// OR_{C' in T(C)} fr1[C'] with T(C) = { sort(n_1..n_K) : n_i in N(c_i) }
+// (AuxGraph.h:194-254). That OR is a K-fold product of per-cop
+// neighbourhoods, so it is taken one cop at a time: stage i turns
+//   H_{i-1}[A'][B'] = OR over the first i-1 cops already moved
+// into H_i[A][B], where A is the multiset of the i moved cops, B of the K-i
+// others (H_0 = fr1, H_K = candidates of C), by
+//   H_i[A][B] = OR_{n in N(max A)} H_{i-1}[A \ {max A}][B + {n}].
+// H_i is symmetric within A and within B, so rows are indexed by the pair of
+// multiset ranks (lexicographic, AuxGraph.h:117-177 order) and the stages
+// read O(sum |H_i| * degree) rows instead of O(C * |T|) random ones; for a
+// fixed A the reads for increasing B are increasing rows of H_{i-1}.
+// ---------------------------------------------------------------------------
+// rank of a sorted j-tuple over [0, n): C(M,j) - 1 - sum_i C(M-1-(t_i+i), j-i), M = n+j-1
+__device__ __forceinline__ uint32_t rank_j(const uint32_t* t, int j, int n, const uint32_t* binom, int Mimg) {
+    const int M = n + j - 1;
+    uint32_t s = 0;
+    for (int i = 0; i < j; ++i) s += binom[(j - i) * (Mimg + 1) + (M - 1 - static_cast<int>(t[i]) - i)];
+    return binom[j * (Mimg + 1) + M] - 1 - s;
+}
+
+__device__ __forceinline__ void unrank_j(uint32_t rank, int j, int n, const uint32_t* binom, int Mimg, uint32_t* out) {
+    const int M = n + j - 1;
+    uint32_t s = binom[j * (Mimg + 1) + M] - 1 - rank;
+    int xmax = M - 1;
+    for (int i = 0; i < j; ++i) {
+        const uint32_t* row = binom + (j - i) * (Mimg + 1);
+        int lo = 0, hi = xmax;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (row[mid] <= s) lo = mid; else hi = mid - 1;
+        }
+        s -= row[lo];
+        out[i] = static_cast<uint32_t>(M - 1 - lo - i);
+        xmax = lo - 1;
+    }
+}
+
+struct SepStage {
+    const uint64_t* in;   // H_{i-1}, rows [cnt(i-1) * cnt(K-i+1)]
+    uint64_t* out;        // H_i, rows [cnt(i) * cnt(K-i)] (stage K: nx0)
+    uint32_t cnt_b;       // cnt(K-i): B-multisets of H_i
+    uint32_t cnt_b_in;    // cnt(K-i+1): B-multisets of H_{i-1}
+    uint64_t rows;        // rows of H_i
+    int32_t stage;        // i, 1..K
+};
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a, SepStage st) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
+    load_image(lds, gv.image, gv.image_bytes);
+    const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
+    const uint16_t* out_off = reinterpret_cast<const uint16_t*>(lds + gv.off_out_off);
+    const uint8_t* out_nbr = lds + gv.off_out_nbr;
+    const int W = gv.W, n = gv.n, i = st.stage, jb = K - i;
+    uint64_t full[4];
+    for (int q = 0; q < 4; ++q) {
+        const int bits = min(64, max(0, n - 64 * q));
+        full[q] = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+    }
+    for (uint64_t o = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; o < st.rows;
+         o += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        // rows < 2^32 here (alloc_pull caps the intermediates): 32-bit division
+        const uint32_t o32 = static_cast<uint32_t>(o);
+        const uint32_t ra = o32 / st.cnt_b, rb = o32 - ra * st.cnt_b;
+        uint32_t A[K], B[K + 1];
+        unrank_j(ra, i, n, binom, gv.M, A);
+        unrank_j(rb, jb, n, binom, gv.M, B);
+        const uint32_t ra_in = rank_j(A, i - 1, n, binom, gv.M);  // A without its largest element
+        const uint32_t c = A[i - 1];
+        uint64_t acc[4] = {0, 0, 0, 0};
+        const int beg = out_off[c], end = out_off[c + 1];
+        for (int e = beg; e < end; ++e) {
+            const uint32_t v = out_nbr[e];
+            // B + {v}, sorted
+            uint32_t M2[K + 1];
+            int pos = 0;
+            bool placed = false;
+            for (int t = 0; t < jb; ++t) {
+                if (!placed && v <= B[t]) { M2[pos++] = v; placed = true; }
+                M2[pos++] = B[t];
+            }
+            if (!placed) M2[pos++] = v;
+            const uint64_t r_in = static_cast<uint64_t>(ra_in) * st.cnt_b_in + rank_j(M2, jb + 1, n, binom, gv.M);
+            for (int q = 0; q < W; ++q) acc[q] |= st.in[r_in * W + q];
+        }
+        if (i < K) {
+            for (int q = 0; q < W; ++q) st.out[o * W + q] = acc[q];
+        } else {  // candidates of configuration ra: only states not won yet
+            for (int q = 0; q < W; ++q) st.out[o * W + q] = acc[q] & ~a.won0[o * W + q] & full[q];
+        }
+    }
+}
+
+// After a pull, the frontier rows still hold their depth-d robber bits;
+// clear them before the robber step writes depth d+1 into its rows.
+__global__ __launch_bounds__(kBlock) void k_clear_frontier(LevelArgs a, int32_t W) {
+    if (a.ctl->done || !a.ctl->pull) return;
+    const uint32_t cur = a.ctl->level & 1;
+    const uint64_t n = static_cast<uint64_t>(a.ctl->cnt1[cur]) * W;
+    const uint32_t* list = a.list1[cur];
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        a.fr1[static_cast<uint64_t>(list[i / W]) * W + i % W] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -628,6 +761,7 @@ __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
         ctl->lvl[d + 1][1] = s1;
         ctl->cnt0[nxt] = t0;
         ctl->cnt1[nxt] = t1;
+        ctl->pull = choose_pull(t1, a.pull_rows);
         ctl->level = d + 1;
         if ((t0 == 0 && t1 == 0) || d + 1 >= kMaxRound - 1) ctl->done = 1;
     }
@@ -658,6 +792,7 @@ __global__ __launch_bounds__(kScanThreads) void k_settle_init(LevelArgs a) {
         ctl->lvl[1][1] = s1;
         ctl->cnt0[1] = t0;
         ctl->cnt1[1] = t1;
+        ctl->pull = choose_pull(t1, a.pull_rows);
         ctl->level = 1;
         ctl->done = (t0 == 0 && t1 == 0) ? 1u : 0u;
     }
@@ -704,6 +839,118 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
 }
 
 // ---------------------------------------------------------------------------
+// exclusive scan of per-configuration counts into 64-bit CSR heads
+// ---------------------------------------------------------------------------
+constexpr int kScanTile = kScanThreads * 8;
+
+__device__ unsigned long long block_exclusive_scan(unsigned long long v, unsigned long long& total) {
+    __shared__ unsigned long long sh[kScanThreads];
+    const uint32_t t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
+        const unsigned long long x = t >= off ? sh[t - off] : 0ull;
+        __syncthreads();
+        sh[t] += x;
+        __syncthreads();
+    }
+    total = sh[kScanThreads - 1];
+    const unsigned long long r = sh[t] - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_tile_sums(const uint32_t* cnt, uint64_t n,
+                                                            unsigned long long* tiles) {
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * 8ull;
+    unsigned long long v = 0;
+    for (int i = 0; i < 8; ++i)
+        if (base + i < n) v += cnt[base + i];
+    unsigned long long total;
+    block_exclusive_scan(v, total);
+    if (threadIdx.x == 0) tiles[blockIdx.x] = total;
+}
+
+// Scans the tile sums in place and stores the grand total as heads[n].
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(unsigned long long* tiles, uint32_t ntiles,
+                                                             unsigned long long* heads, uint64_t n) {
+    unsigned long long carry = 0;
+    for (uint32_t b0 = 0; b0 < ntiles; b0 += kScanThreads) {
+        const uint32_t i = b0 + threadIdx.x;
+        const unsigned long long v = i < ntiles ? tiles[i] : 0ull;
+        unsigned long long total;
+        const unsigned long long ex = block_exclusive_scan(v, total);
+        if (i < ntiles) tiles[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) heads[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_tile_apply(const uint32_t* cnt, uint64_t n,
+                                                             const unsigned long long* tiles,
+                                                             unsigned long long* heads) {
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * 8ull;
+    unsigned long long v = 0;
+    for (int i = 0; i < 8; ++i)
+        if (base + i < n) v += cnt[base + i];
+    unsigned long long total;
+    unsigned long long run = tiles[blockIdx.x] + block_exclusive_scan(v, total);
+    for (int i = 0; i < 8; ++i) {
+        if (base + i < n) {
+            heads[base + i] = run;
+            run += cnt[base + i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// cop turn, push over the precomputed predecessor CSR (built once by
+// cr_build_aux_graph): one wavefront per frontier row streams its
+// predecessor slice and ORs the row's robber mask into their nx0 words
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_cop_push_csr(GraphView gv, LevelArgs a,
+                                                         const unsigned long long* heads,
+                                                         const uint32_t* csr) {
+    if (a.ctl->done || a.ctl->pull) return;
+    const uint32_t cur = a.ctl->level & 1;
+    const int lane = threadIdx.x & 63;
+    const int W = gv.W;
+    const uint32_t count = a.ctl->cnt1[cur];
+    const uint32_t* list = a.list1[cur];
+    for (uint32_t it = blockIdx.x * kWavesPerBlock + wave_id(); it < count; it += gridDim.x * kWavesPerBlock) {
+        const uint32_t row = list[it];
+        uint64_t m[4] = {0, 0, 0, 0};
+        for (int w = 0; w < W; ++w) m[w] = a.fr1[static_cast<uint64_t>(row) * W + w];
+        const unsigned long long beg = heads[row], end = heads[row + 1];
+        for (unsigned long long i = beg + lane; i < end; i += 256) {
+            uint64_t pr[4];
+            bool keep[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                keep[u] = i + 64 * u < end;
+                pr[u] = keep[u] ? csr[i + 64 * u] : 0u;
+            }
+            for (int w = 0; w < W; ++w) {
+                if (m[w] == 0) continue;
+                uint64_t mw[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) mw[u] = keep[u] ? m[w] : 0ull;
+                if (a.filter) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (mw[u]) mw[u] &= ~a.won0[pr[u] * W + w];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (mw[u]) atomicOr(reinterpret_cast<unsigned long long*>(a.nx0 + pr[u] * W + w),
+                                        static_cast<unsigned long long>(mw[u]));
+            }
+        }
+        if (lane < W) a.fr1[static_cast<uint64_t>(row) * W + lane] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // verdict (k_cops_visibility.cpp:301-354): universal rows -> min worst depth,
 // lowest cId on ties; packed (worst << 32 | cId) atomicMin
 // ---------------------------------------------------------------------------
@@ -742,10 +989,19 @@ __global__ __launch_bounds__(kBlock) void k_verdict(GraphView gv, LevelArgs a, i
 // ---------------------------------------------------------------------------
 // mode 0: retro-edge count of a solved table over predecessor sets (in-lists);
 // mode 1: total forward team moves sum |T(C)| (out-lists), the reference's
-//         transitions.size() (AuxGraph.h:258-274).
+//         transitions.size() (AuxGraph.h:258-274);
+// mode 2: |pred(C)| of every configuration into rowcnt[C];
+// mode 3: the distinct predecessors of C into csr[heads[C] ..) (any order).
+struct MoveArgs {
+    uint32_t* rowcnt;
+    const unsigned long long* heads;
+    uint32_t* csr;
+};
+
 template <int K>
 __global__ __launch_bounds__(kWave) void k_count_retro(GraphView gv, LevelArgs a, uint32_t slots,
-                                                       uint32_t shift, int32_t mode) {
+                                                       uint32_t shift, int32_t mode, MoveArgs mv) {
+    __shared__ uint32_t fill_pos;
     extern __shared__ __align__(16) uint8_t lds[];
     load_image(lds, gv.image, gv.image_bytes);
     // slot = (row + 1) << 32 | rank: a slot tagged with another row counts as empty
@@ -753,8 +1009,8 @@ __global__ __launch_bounds__(kWave) void k_count_retro(GraphView gv, LevelArgs a
     const uint32_t* rank_tab = reinterpret_cast<const uint32_t*>(lds);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const uint16_t* in_off = reinterpret_cast<const uint16_t*>(lds + gv.off_in_off);
-    const uint16_t* mv_off = reinterpret_cast<const uint16_t*>(lds + (mode ? gv.off_out_off : gv.off_in_off));
-    const uint8_t* mv_nbr = lds + (mode ? gv.off_out_nbr : gv.off_in_nbr);
+    const uint16_t* mv_off = reinterpret_cast<const uint16_t*>(lds + (mode == 1 ? gv.off_out_off : gv.off_in_off));
+    const uint8_t* mv_nbr = lds + (mode == 1 ? gv.off_out_nbr : gv.off_in_nbr);
     const int lane = threadIdx.x;
     for (uint32_t i = lane; i < slots; i += 64) hs[i] = 0;
     __syncthreads();
@@ -771,6 +1027,9 @@ __global__ __launch_bounds__(kWave) void k_count_retro(GraphView gv, LevelArgs a
         }
         const unsigned long long tag = static_cast<unsigned long long>(row + 1) << 32;
         uint32_t distinct = 0;
+        if (lane == 0) fill_pos = 0;
+        __syncthreads();
+        uint32_t* out = mode == 3 ? mv.csr + mv.heads[row] : nullptr;
         for (uint32_t t = lane; t < P; t += 64) {
             uint32_t tup[K];
             if (!product_tuple<K>(t, cops, deg, base, mv_nbr, tup)) continue;
@@ -782,17 +1041,24 @@ __global__ __launch_bounds__(kWave) void k_count_retro(GraphView gv, LevelArgs a
                 if (cur == key) break;
                 if ((cur >> 32) != (tag >> 32)) {
                     const unsigned long long seen = atomicCAS(&hs[h], cur, key);
-                    if (seen == cur) { ++distinct; break; }
+                    if (seen == cur) {
+                        ++distinct;
+                        if (mode == 3) out[atomicAdd(&fill_pos, 1u)] = pr;
+                        break;
+                    }
                     continue;  // lost the race: re-examine this slot
                 }
                 h = (h + 1) & (slots - 1);
             }
         }
         for (int off = 32; off > 0; off >>= 1) distinct += __shfl_xor(distinct, off);
+        if (mode == 2 && lane == 0) mv.rowcnt[row] = distinct;
         if (mode) {
             e_cop += distinct;
+            __syncthreads();  // fill_pos is reset for the next row
             continue;
         }
+        __syncthreads();
         uint32_t rob = 0, cop_e = 0, w = 0;
         for (int r = lane; r < gv.n; r += 64) {
             const uint8_t b0 = a.table[(static_cast<uint64_t>(row) * gv.n + r) * 2];
@@ -862,6 +1128,10 @@ struct cr_solver {
     uint32_t* d_gcnt[2] = {nullptr, nullptr};                           // per-group counts / offsets
     unsigned long long* d_gstates[2] = {nullptr, nullptr};              // per-group state counts
     unsigned long long* d_gcaps = nullptr;
+    unsigned long long* d_heads = nullptr;  // predecessor CSR (C+1 heads, u32 ranks)
+    uint32_t* d_csr = nullptr;
+    uint64_t csr_entries = 0;
+    double csr_build_ms = 0;
     Ctl* d_ctl = nullptr;
     uint8_t* d_image = nullptr;
     uint64_t* d_outmask = nullptr;
@@ -870,6 +1140,11 @@ struct cr_solver {
     GraphView gv{};
     // tuning knobs (cr_set_tuning)
     int32_t filter = 1;
+    int32_t use_csr = 0;  // precompute predecessor lists (off by default: no gain measured)
+    uint64_t pull_scale = 500;    // pull threshold scale (permille of the estimated break-even)
+    uint64_t* d_sep[2] = {nullptr, nullptr};  // factorised-pull intermediates H_i
+    uint64_t ms_count[CR_MAX_K + 1] = {};     // C(n+j-1, j): multisets of size j
+    uint32_t pull_rows = 0xffffffffu;         // frontier rows above which a level pulls
     uint32_t push_blocks = 2048;
     uint32_t row_blocks = 2048;
     int32_t levels_per_sync = 8;
@@ -910,6 +1185,14 @@ struct cr_solver {
         }
         hipFree(d_gcaps);
         d_gcaps = nullptr;
+        hipFree(d_heads);
+        hipFree(d_csr);
+        d_heads = nullptr;
+        d_csr = nullptr;
+        hipFree(d_sep[0]);
+        hipFree(d_sep[1]);
+        d_sep[0] = d_sep[1] = nullptr;
+        csr_entries = 0;
         hipFree(d_ctl);
         hipFree(d_image);
         hipFree(d_outmask);
@@ -1034,9 +1317,37 @@ struct LaunchInit {
 template <int K>
 struct LaunchPush {
     static int run(cr_solver* s, LevelArgs a) {
+        if (s->d_csr) {
+            hipLaunchKernelGGL(k_cop_push_csr, dim3(s->push_blocks), dim3(kBlock), 0, s->stream, s->gv, a, s->d_heads,
+                               s->d_csr);
+            return 0;
+        }
         if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_cop_push<K>), s->gv.image_bytes)) return rc;
         hipLaunchKernelGGL(k_cop_push<K>, dim3(s->push_blocks), dim3(kBlock), s->gv.image_bytes, s->stream,
                            s->gv, a);
+        return 0;
+    }
+};
+
+template <int K>
+struct LaunchPull {
+    static int run(cr_solver* s, LevelArgs a) {
+        if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_sep_stage<K>), s->gv.image_bytes)) return rc;
+        const uint64_t* in = s->d_fr1;
+        for (int i = 1; i <= K; ++i) {
+            SepStage st{};
+            st.stage = i;
+            st.in = in;
+            st.out = i == K ? s->d_nx0 : s->d_sep[(i - 1) & 1];
+            st.cnt_b = static_cast<uint32_t>(s->ms_count[K - i]);
+            st.cnt_b_in = static_cast<uint32_t>(s->ms_count[K - i + 1]);
+            st.rows = s->ms_count[i] * s->ms_count[K - i];
+            const uint64_t blocks = std::min<uint64_t>((st.rows + kBlock - 1) / kBlock, 16384);
+            hipLaunchKernelGGL(k_sep_stage<K>, dim3(std::max<uint64_t>(blocks, 1)), dim3(kBlock), s->gv.image_bytes,
+                               s->stream, s->gv, a, st);
+            in = st.out;
+        }
+        hipLaunchKernelGGL(k_clear_frontier, dim3(1024), dim3(kBlock), 0, s->stream, a, s->W);
         return 0;
     }
 };
@@ -1068,14 +1379,15 @@ struct LaunchVerdict {
 
 template <int K>
 struct LaunchRetro {
-    static int run(cr_solver* s, LevelArgs a, uint32_t slots, int32_t mode) {
+    static int run(cr_solver* s, LevelArgs a, uint32_t slots, int32_t mode, MoveArgs mv) {
         const uint32_t lds = ((s->gv.image_bytes + 15) & ~15u) + slots * 8;
         uint32_t shift = 32;
         for (uint32_t v = slots; v > 1; v >>= 1) --shift;
         hipFuncSetAttribute(reinterpret_cast<const void*>(k_count_retro<K>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(s->C, 8192));
-        hipLaunchKernelGGL(k_count_retro<K>, dim3(blocks), dim3(kWave), lds, s->stream, s->gv, a, slots, shift, mode);
+        hipLaunchKernelGGL(k_count_retro<K>, dim3(blocks), dim3(kWave), lds, s->stream, s->gv, a, slots, shift, mode,
+                           mv);
         return 0;
     }
 };
@@ -1095,6 +1407,7 @@ LevelArgs level_args(cr_solver* s) {
     a.c1 = Compact{s->d_flags[1], s->d_gcnt[1], s->d_gstates[1]};
     a.gcaps = s->d_gcaps;
     a.nbatch = s->nbatch();
+    a.pull_rows = s->pull_rows;
     a.ctl = s->d_ctl;
     a.out_mask = s->d_outmask;
     a.filter = s->filter;
@@ -1181,6 +1494,51 @@ int do_load(cr_solver* s, int32_t n, const uint8_t* adj, int32_t k, int32_t p) {
     return 0;
 }
 
+int build_csr(cr_solver* s);
+
+// Break-even of the two cop-turn directions: a push costs about |pred| random
+// relaxations per frontier row, the factorised pull about sum_i |H_i| * deg
+// row reads per level regardless of the frontier.
+void set_pull_threshold(cr_solver* s) {
+    if (!s->d_sep[0]) {
+        s->pull_rows = 0xffffffffu;
+        return;
+    }
+    const int n = s->g.n, K = s->k;
+    uint64_t edges = 0;
+    for (int v = 0; v < n; ++v) edges += s->g.out[v].size();
+    const double deg = static_cast<double>(edges) / n;
+    double pull_reads = 0;
+    for (int i = 1; i <= K; ++i) pull_reads += static_cast<double>(s->ms_count[i]) * s->ms_count[K - i] * deg;
+    const double pred = std::pow(std::max(1.0, static_cast<double>(edges) / n), K);
+    const double rows = pull_reads / pred * static_cast<double>(s->pull_scale) / 1000.0;
+    s->pull_rows = rows >= 4.0e9 ? 0xffffffffu : static_cast<uint32_t>(rows);
+}
+
+// Intermediates of the factorised pull: H_1..H_{K-1} alternate between two
+// buffers sized for the largest of each parity. Skipped (the push then runs
+// every level) when they do not fit next to the state space.
+int alloc_pull(cr_solver* s) {
+    const int n = s->g.n, K = s->k;
+    for (int j = 0; j <= K; ++j) s->ms_count[j] = cr::binom(n + j - 1, j);
+    uint64_t need[2] = {0, 0};
+    for (int i = 1; i < K; ++i) {
+        const unsigned __int128 rows = static_cast<unsigned __int128>(s->ms_count[i]) * s->ms_count[K - i];
+        if (rows >= (static_cast<unsigned __int128>(1) << 32)) return 0;
+        need[(i - 1) & 1] = std::max<uint64_t>(need[(i - 1) & 1], static_cast<uint64_t>(rows) * s->W * 8);
+    }
+    for (int j = 0; j <= K; ++j)
+        if (s->ms_count[j] >= 0xffffffffull) return 0;
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    if (need[0] + need[1] + (1ull << 30) > freeb) return 0;
+    int rc = 0;
+    if (!(s->d_sep[0] = dev_alloc<uint64_t>(std::max<uint64_t>(need[0] / 8, 1), s->device_bytes, rc))) return rc;
+    if (!(s->d_sep[1] = dev_alloc<uint64_t>(std::max<uint64_t>(need[1] / 8, 1), s->device_bytes, rc))) return rc;
+    set_pull_threshold(s);
+    return 0;
+}
+
 int do_build(cr_solver* s) {
     if (s->stage < kLoaded) return fail(CR_ERR_ORDER, "buildAuxGraph before loadGraphFile");
     s->release();
@@ -1220,6 +1578,8 @@ int do_build(cr_solver* s) {
     if (!(s->d_ctl = dev_alloc<Ctl>(1, s->device_bytes, rc))) return rc;
     if (!s->h_ctl) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_ctl), sizeof(Ctl), hipHostMallocDefault));
     if ((rc = upload_graph(s)) != 0) return rc;
+    if (s->use_csr && (rc = build_csr(s)) != 0) return rc;
+    if ((rc = alloc_pull(s)) != 0) return rc;
     s->stage = kBuilt;
     return 0;
 }
@@ -1257,6 +1617,8 @@ int do_init(cr_solver* s, uint64_t* base) {
     hipEventDestroy(e1);
     s->stats = cr_stats{};
     s->stats.init_ms = ms;
+    s->stats.pred_entries = s->csr_entries;
+    s->stats.build_ms = s->csr_build_ms;
     s->stats.base_captures = s->h_ctl->lvl[0][0];
     if (base) *base = s->stats.base_captures;
     s->stage = kInitialized;
@@ -1299,6 +1661,9 @@ int do_loop(cr_solver* s) {
             timers.emplace_back(new Timer(s, "cop_push"));
             if ((rc = dispatch_k<LaunchPush>(s->k, s, a)) != 0) return rc;
             timers.back()->stop();
+            timers.emplace_back(new Timer(s, "cop_pull"));
+            if (s->d_sep[0] && (rc = dispatch_k<LaunchPull>(s->k, s, a)) != 0) return rc;
+            timers.back()->stop();
             timers.emplace_back(new Timer(s, "robber_step"));
             if ((rc = dispatch_w<LaunchRobber>(s->W, s, a)) != 0) return rc;
             timers.back()->stop();
@@ -1332,10 +1697,10 @@ int do_loop(cr_solver* s) {
     return 0;
 }
 
-int count_moves(cr_solver* s, int32_t mode, unsigned long long* r) {
+int count_moves(cr_solver* s, int32_t mode, unsigned long long* r, MoveArgs mv = MoveArgs{}) {
     HIP_TRY(hipSetDevice(s->device));
     uint64_t prod = 1;
-    const int deg = std::max(1, mode ? s->g.max_out : s->g.max_in);
+    const int deg = std::max(1, mode == 1 ? s->g.max_out : s->g.max_in);
     for (int i = 0; i < s->k; ++i) prod *= static_cast<uint64_t>(deg);
     uint32_t slots = 64;
     while (slots < 2 * prod) slots <<= 1;
@@ -1345,10 +1710,75 @@ int count_moves(cr_solver* s, int32_t mode, unsigned long long* r) {
     unsigned long long zero[3] = {0, 0, 0};
     HIP_TRY(hipMemcpyAsync(s->d_ctl->retro, zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
     LevelArgs a = level_args(s);
-    if (int rc = dispatch_k<LaunchRetro>(s->k, s, a, slots, mode)) return rc;
+    if (int rc = dispatch_k<LaunchRetro>(s->k, s, a, slots, mode, mv)) return rc;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(r, s->d_ctl->retro, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+// Predecessor lists of every configuration, deduplicated, as a CSR of u32
+// ranks (the device form of AuxGraph::createTransitions, AuxGraph.h:179-275,
+// over in-edges). Skipped -- the push then enumerates team moves on the fly --
+// when it does not fit in device memory or the dedup table exceeds LDS.
+int build_csr(cr_solver* s) {
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    uint64_t tmp_bytes = 0;
+    int rc = 0;
+    uint32_t* cnt = dev_alloc<uint32_t>(s->C, tmp_bytes, rc);
+    if (!cnt) return rc;
+    unsigned long long r[3];
+    rc = count_moves(s, 2, r, MoveArgs{cnt, nullptr, nullptr});
+    if (rc == CR_ERR_UNSUPPORTED) {  // dedup table does not fit in LDS: no CSR
+        hipFree(cnt);
+        g_err[0] = '\0';
+        return 0;
+    }
+    if (rc) {
+        hipFree(cnt);
+        return rc;
+    }
+    const uint64_t total = r[0];
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    const uint64_t ntiles = (s->C + kScanTile - 1) / kScanTile;
+    const uint64_t need = total * 4 + (s->C + 1) * 8 + ntiles * 8 + (256ull << 20);
+    if (need > freeb) {
+        hipFree(cnt);
+        return 0;
+    }
+    unsigned long long* tiles = dev_alloc<unsigned long long>(ntiles, tmp_bytes, rc);
+    if (!tiles) { hipFree(cnt); return rc; }
+    if (!(s->d_heads = dev_alloc<unsigned long long>(s->C + 1, s->device_bytes, rc))) {
+        hipFree(cnt);
+        hipFree(tiles);
+        return rc;
+    }
+    hipLaunchKernelGGL(k_tile_sums, dim3(ntiles), dim3(kScanThreads), 0, s->stream, cnt, s->C, tiles);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s->stream, tiles, static_cast<uint32_t>(ntiles),
+                       s->d_heads, s->C);
+    hipLaunchKernelGGL(k_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, s->stream, cnt, s->C, tiles, s->d_heads);
+    HIP_TRY(hipGetLastError());
+    if (!(s->d_csr = dev_alloc<uint32_t>(total, s->device_bytes, rc))) {
+        hipFree(cnt);
+        hipFree(tiles);
+        return rc;
+    }
+    rc = count_moves(s, 3, r, MoveArgs{nullptr, s->d_heads, s->d_csr});
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    hipFree(cnt);
+    hipFree(tiles);
+    if (rc) return rc;
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    s->csr_entries = total;
+    s->csr_build_ms = ms;
     return 0;
 }
 
@@ -1568,12 +1998,18 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
         case CR_TUNE_PUSH_FILTER: s->filter = value != 0; return 0;
         case CR_TUNE_PUSH_BLOCKS:
             if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "push blocks %lld", static_cast<long long>(value));
-            s->push_blocks = static_cast<uint32_t>(value);
+            s->push_blocks = static_cast<uint32_t>((value + kXcds - 1) / kXcds * kXcds);  // whole XCD rounds
             return 0;
         case CR_TUNE_ROW_BLOCKS:
             if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "row blocks %lld", static_cast<long long>(value));
             s->row_blocks = static_cast<uint32_t>(value);
             return 0;
+        case CR_TUNE_PULL_SCALE:
+            if (value < 0) return fail(CR_ERR_ARG, "pull scale %lld", static_cast<long long>(value));
+            s->pull_scale = static_cast<uint64_t>(value);
+            set_pull_threshold(s);
+            return 0;
+        case CR_TUNE_PRED_CSR: s->use_csr = value != 0; if (s->stage > kLoaded) s->stage = kLoaded; return 0;
         case CR_TUNE_LEVELS_PER_SYNC:
             if (value < 1 || value > 128) return fail(CR_ERR_ARG, "levels per sync %lld", static_cast<long long>(value));
             s->levels_per_sync = static_cast<int32_t>(value);

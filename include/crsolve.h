@@ -87,6 +87,8 @@ typedef struct {
     int32_t n_levels;       /* device levels / passes executed */
     double init_ms;         /* device time of initializeCaptures */
     double loop_ms;         /* device time of mainLoop */
+    uint64_t pred_entries;  /* predecessor-list entries built by cr_build_aux_graph (0: none) */
+    double build_ms;        /* device time to build them */
 } cr_stats;
 
 /* ---- lifetime -------------------------------------------------------- */
@@ -151,7 +153,11 @@ enum cr_tuning {
     CR_TUNE_PUSH_FILTER = 1,    /* 1: cop push skips targets already won (default 1) */
     CR_TUNE_PUSH_BLOCKS = 2,    /* workgroups of the cop-push kernel (default 2048) */
     CR_TUNE_ROW_BLOCKS = 3,     /* workgroups of the row kernels (default 2048) */
-    CR_TUNE_LEVELS_PER_SYNC = 4 /* depth levels enqueued per host synchronisation (default 4) */
+    CR_TUNE_LEVELS_PER_SYNC = 4, /* depth levels enqueued per host synchronisation (default 8) */
+    CR_TUNE_PRED_CSR = 5,        /* 1: cr_build_aux_graph precomputes predecessor lists (default 0) */
+    CR_TUNE_PULL_SCALE = 6       /* a level gathers (factorised pull) instead of pushing when its robber
+                                    frontier exceeds value/1000 times the estimated break-even row count;
+                                    0: always pull, >= 10^9: never (default 500) */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:
