@@ -5,22 +5,28 @@ Workload (default): Erdos-Renyi G(200, 0.03), first connected seed >= 1
 (p = 1): C(202, 3) = 1,353,400 configurations, 5.41e8 states. A step is one
 full solve with the tables resident in HBM: initializeCaptures + mainLoop
 (every depth level until the fixed point). The device tables are allocated
-once before the timed region.
+once, before the timed region.
 
 Unit of work: a retro-edge, one predecessor relaxation of the push
 formulation, E_retro = E_cop + E_rob (won robber-turn states x |pred(C')| +
 won cop-turn states x in-degree(v), depth <= 125). value = E_retro * steps /
-elapsed, in GTEPS.
+elapsed, in GTEPS. The relaxations are counted from the solved table; how a
+level performs them (push, factorised pull, or the closed form of depth 1)
+does not change the count.
 
-Also reported:
-  roofline     the dominant kernel's algorithmic bytes (0.5 B per cop
-               retro-edge for cop_push: 2 bits read + 2 bits written per
-               relaxation; 2 B per robber retro-edge for robber_step) over its
-               own HIP-event time, against 8 TB/s.
-  cpu_baseline the reference solver (oracle/_ref, built from the reference's
-               sources) on a bounded sample -- the same graph at k = 2 -- on
-               one host core, as GTEPS; plus the multithreaded oracle port on
-               the full workload when --cpu-port is given.
+roofline: the dominant kernel's algorithmic bytes over its own HIP-event time
+(events recorded on the solver's stream around every launch), against 8 TB/s.
+Algorithmic bytes per unit (scalar-record model): 0.5 B per cop retro-edge
+(2-bit outcome read + written) for the cop-turn kernels of the levels they
+served, 2 B per robber retro-edge (8-bit escape counter read + written) for
+the robber step, 1.25 B per newly won cop-turn state (depth byte + frontier
+bit) for the finaliser.
+
+cpu_baseline: the reference solver (oracle/_ref/k_cops_visibility, built from
+the reference's own sources) on one host core, on a bounded sample of the
+workload -- the same graph with one cop fewer -- as GTEPS of that run.
+cpu_port: the multithreaded level-synchronous CPU restatement of the oracle
+on the full workload, on the host's cores.
 """
 import argparse
 import json
@@ -42,8 +48,21 @@ WORKLOADS = {
     "planar200_k4": ("planar200_e400_s1", 4),
     "sy_all_k2": ("scotlandyard-all", 2),
 }
-# algorithmic bytes per unit of work, by kernel
-BYTES_PER_UNIT = {"cop_push": ("e_cop", 0.5), "robber_step": ("e_rob", 2.0)}
+# solver timer name -> device kernel name in rocprofv3 summaries
+KERNEL_NAMES = {"cop_push": "k_cop_push", "sep_stage": "k_sep_stage", "robber_step": "k_robber_step",
+                "cop_finalize": "k_cop_finalize", "clear_frontier": "k_clear_frontier"}
+
+
+def measured_traffic(workload, timer_name):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 PMC summary
+    (profiles/<workload>_traffic.json, FETCH_SIZE/WRITE_SIZE scaled by the copy calibration)."""
+    path = os.path.join(REPO, "profiles", f"{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    k = data.get("kernels", {}).get(KERNEL_NAMES.get(timer_name, timer_name))
+    return k["hbm_bytes"] if k else None
 
 
 def matrix_path(stem):
@@ -101,7 +120,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="er200_k3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-port", type=int, default=0, help="also time the oracle port on N threads")
+    ap.add_argument("--cpu-port", type=int, default=-1,
+                    help="threads for the oracle-port timing (default: min(16, cores); 0 = skip)")
     ap.add_argument("--check", action="store_true", help="compare the table with the oracle port")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (rocprof counter calibration)")
@@ -126,8 +146,6 @@ def main():
     for _ in range(args.warmup):
         s.initialize_captures()
         s.main_loop()
-    e_cop, e_rob, won = s.retro_edges()
-    e_retro = e_cop + e_rob
     s.set_kernel_timing(True)
     kt_total = {}
     torch.cuda.synchronize()
@@ -141,23 +159,36 @@ def main():
             a[1] += launches
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    # the work count comes from the table the last timed step solved (untimed)
+    e_cop, e_rob, won = s.retro_edges()
+    e_retro = e_cop + e_rob
     verdict = s.find_final_result()
     stats = s.stats()
     ms_per_step = elapsed * 1e3 / args.steps
     value = e_retro * args.steps / elapsed / 1e9
 
-    units = {"e_cop": e_cop, "e_rob": e_rob}
+    e_cop_d, e_rob_d, pulled = s.work_profile()
+    lv = s.level_counts()
+    push_units = sum(int(e_cop_d[d]) for d in range(1, 127) if not pulled[d])
+    pull_units = sum(int(e_cop_d[d]) for d in range(1, 127) if pulled[d])
+    alg_bytes = {
+        "cop_push": 0.5 * push_units,
+        "sep_stage": 0.5 * pull_units,
+        "robber_step": 2.0 * sum(int(x) for x in e_rob_d[1:]),
+        "cop_finalize": 1.25 * sum(int(lv[d, 0]) for d in range(2, 127)),
+    }
     dominant = max(kt_total, key=lambda n: kt_total[n][0])
     rl = None
-    if dominant in BYTES_PER_UNIT:
-        unit_name, per = BYTES_PER_UNIT[dominant]
+    if dominant in alg_bytes:
         ms, launches = kt_total[dominant]
-        bytes_per_launch = units[unit_name] * per * args.steps / launches
+        bytes_per_launch = alg_bytes[dominant] * args.steps / launches
         avg_s = ms / 1e3 / launches
         achieved = bytes_per_launch / avg_s
         rl = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-              "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dominant,
-              "avg_launch_us": avg_s * 1e6, "launches_per_step": launches // args.steps}
+              "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.workload, dominant),
+              "kernel": KERNEL_NAMES.get(dominant, dominant),
+              "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
+              "launches_per_step": launches // args.steps}
     b_alg = 2 * e_rob + 0.5 * e_cop + won * 1.25 + info.num_states * (0.25 + 1) + info.num_states / 2
     out = {
         "metric": "retro-edges/s (GTEPS)",
@@ -177,7 +208,8 @@ def main():
                    "e_cop": e_cop, "e_rob": e_rob, "won_states": won, "levels": stats.n_levels,
                    "passes": len(passes), "verdict": "WIN" if verdict else "LOSS"},
         "roofline": rl,
-        "solve_alg_bytes_per_s": b_alg / (elapsed / args.steps),
+        "solve_roofline": {"alg_bytes": b_alg, "achieved_GBps": b_alg / (elapsed / args.steps) / 1e9,
+                           "frac": b_alg / (elapsed / args.steps) / HBM_PEAK},
         "kernels_ms_per_step": {n: v[0] / args.steps for n, v in kt_total.items()},
         "cpu_baseline": None,
     }
@@ -191,6 +223,8 @@ def main():
             t.main_loop()
             se = sum(t.retro_edges()[:2])
         out["cpu_baseline"] = cpu_reference_sample(stem, sk, se)
+    if args.cpu_port < 0:
+        args.cpu_port = 0 if args.no_cpu_baseline else min(16, os.cpu_count() or 1)
     if args.cpu_port:
         port, states = cpu_port(stem, k, args.cpu_port)
         out["cpu_port"] = port

@@ -36,7 +36,7 @@ EXPORTS = (
     "cr_load_graph_file", "cr_load_graph_matrix", "cr_set_variant", "cr_build_aux_graph",
     "cr_initialize_captures", "cr_main_loop", "cr_find_final_result", "cr_save_aux_graph",
     "cr_get_info", "cr_get_stats", "cr_copy_states", "cr_copy_configs", "cr_level_counts",
-    "cr_count_retro_edges", "cr_count_transitions", "cr_set_tuning", "cr_set_kernel_timing", "cr_kernel_times", "cr_cache_key",
+    "cr_count_retro_edges", "cr_work_profile", "cr_count_transitions", "cr_set_tuning", "cr_set_kernel_timing", "cr_kernel_times", "cr_cache_key",
 )
 
 
@@ -105,6 +105,7 @@ def lib():
         L.cr_level_counts.argtypes = [vp, u64p]
         L.cr_count_retro_edges.argtypes = [vp, u64p, u64p, u64p]
         L.cr_count_transitions.argtypes = [vp, u64p]
+        L.cr_work_profile.argtypes = [vp, u64p, u64p, u8p]
         L.cr_set_tuning.argtypes = [vp, i32, ctypes.c_int64]
         L.cr_set_kernel_timing.argtypes = [vp, i32]
         L.cr_kernel_times.argtypes = [vp, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
@@ -244,6 +245,14 @@ class Solver:
         _check(lib().cr_count_retro_edges(self._h, ctypes.byref(ec), ctypes.byref(er),
                                           ctypes.byref(w)))
         return int(ec.value), int(er.value), int(w.value)
+
+    def work_profile(self):
+        """(e_cop[127], e_rob[127], pulled[127]) by depth of the solved table."""
+        ec = np.zeros(MAX_ROUND, np.uint64)
+        er = np.zeros(MAX_ROUND, np.uint64)
+        pl = np.zeros(MAX_ROUND, np.uint8)
+        _check(lib().cr_work_profile(self._h, _u64(ec), _u64(er), _u8(pl)))
+        return ec, er, pl
 
     def count_transitions(self):
         t = ctypes.c_uint64()

@@ -51,6 +51,8 @@
 
 namespace {
 
+struct Timer;  // per-launch HIP-event pair (host side)
+
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
@@ -95,7 +97,9 @@ struct Ctl {
     uint32_t cnt0[2];  // per parity: length of the list of configurations with new cop-turn states
     uint32_t cnt1[2];  // ... with new robber-turn states
     uint32_t pull;     // the level's cop turn gathers (pull) instead of pushing
+    uint32_t dir[128]; // per level: 1 = pulled
     unsigned long long lvl[128][2];
+    unsigned long long hist[2][128];  // retro-edges by depth: [0] cop (robber states), [1] robber (cop states)
     unsigned long long verdict;
     unsigned long long retro[3];
 };
@@ -762,6 +766,7 @@ __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
         ctl->cnt0[nxt] = t0;
         ctl->cnt1[nxt] = t1;
         ctl->pull = choose_pull(t1, a.pull_rows);
+        ctl->dir[d + 1] = ctl->pull;
         ctl->level = d + 1;
         if ((t0 == 0 && t1 == 0) || d + 1 >= kMaxRound - 1) ctl->done = 1;
     }
@@ -793,6 +798,7 @@ __global__ __launch_bounds__(kScanThreads) void k_settle_init(LevelArgs a) {
         ctl->cnt0[1] = t0;
         ctl->cnt1[1] = t1;
         ctl->pull = choose_pull(t1, a.pull_rows);
+        ctl->dir[1] = ctl->pull;
         ctl->level = 1;
         ctl->done = (t0 == 0 && t1 == 0) ? 1u : 0u;
     }
@@ -1002,6 +1008,8 @@ template <int K>
 __global__ __launch_bounds__(kWave) void k_count_retro(GraphView gv, LevelArgs a, uint32_t slots,
                                                        uint32_t shift, int32_t mode, MoveArgs mv) {
     __shared__ uint32_t fill_pos;
+    __shared__ unsigned long long hist[2][128];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i >> 7][i & 127] = 0;
     extern __shared__ __align__(16) uint8_t lds[];
     load_image(lds, gv.image, gv.image_bytes);
     // slot = (row + 1) << 32 | rank: a slot tagged with another row counts as empty
@@ -1063,8 +1071,21 @@ __global__ __launch_bounds__(kWave) void k_count_retro(GraphView gv, LevelArgs a
         for (int r = lane; r < gv.n; r += 64) {
             const uint8_t b0 = a.table[(static_cast<uint64_t>(row) * gv.n + r) * 2];
             const uint8_t b1 = a.table[(static_cast<uint64_t>(row) * gv.n + r) * 2 + 1];
-            if (b0 & 1) { ++w; if ((b0 >> 1) <= kMaxRound - 2) cop_e += in_off[r + 1] - in_off[r]; }
-            if (b1 & 1) { ++w; if ((b1 >> 1) <= kMaxRound - 2) ++rob; }
+            if (b0 & 1) {
+                ++w;
+                if ((b0 >> 1) <= kMaxRound - 2) {
+                    const uint32_t e = in_off[r + 1] - in_off[r];
+                    cop_e += e;
+                    atomicAdd(&hist[1][b0 >> 1], static_cast<unsigned long long>(e));
+                }
+            }
+            if (b1 & 1) {
+                ++w;
+                if ((b1 >> 1) <= kMaxRound - 2) {
+                    ++rob;
+                    atomicAdd(&hist[0][b1 >> 1], static_cast<unsigned long long>(distinct));
+                }
+            }
         }
         for (int off = 32; off > 0; off >>= 1) rob += __shfl_xor(rob, off);
         e_cop += static_cast<unsigned long long>(rob) * distinct;
@@ -1080,6 +1101,10 @@ __global__ __launch_bounds__(kWave) void k_count_retro(GraphView gv, LevelArgs a
         atomicAdd(&a.ctl->retro[1], e_rob);
         atomicAdd(&a.ctl->retro[2], won);
     }
+    __syncthreads();
+    if (mode == 0)
+        for (int i = threadIdx.x; i < 256; i += blockDim.x)
+            if (hist[i >> 7][i & 127]) atomicAdd(&a.ctl->hist[i >> 7][i & 127], hist[i >> 7][i & 127]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1162,6 +1187,7 @@ struct cr_solver {
     std::vector<uint64_t> pass_counts;
     cr_stats stats{};
     bool timing = false;
+    std::vector<std::unique_ptr<Timer>>* timers = nullptr;  // events of the chunk in flight
     std::map<std::string, KTime> ktimes;
     std::vector<std::string> korder;
 
@@ -1280,6 +1306,12 @@ struct Timer {
     }
 };
 
+// A timer for the launch about to be enqueued, kept until the chunk syncs.
+Timer* start_timer(cr_solver* s, const char* name) {
+    s->timers->emplace_back(new Timer(s, name));
+    return s->timers->back().get();
+}
+
 // Kernels whose dynamic LDS (the graph image) exceeds the 64 KiB default need
 // the per-function limit raised before launch.
 int raise_lds_limit(const void* fn, uint32_t bytes) {
@@ -1343,11 +1375,15 @@ struct LaunchPull {
             st.cnt_b_in = static_cast<uint32_t>(s->ms_count[K - i + 1]);
             st.rows = s->ms_count[i] * s->ms_count[K - i];
             const uint64_t blocks = std::min<uint64_t>((st.rows + kBlock - 1) / kBlock, 16384);
+            Timer* t = start_timer(s, "sep_stage");
             hipLaunchKernelGGL(k_sep_stage<K>, dim3(std::max<uint64_t>(blocks, 1)), dim3(kBlock), s->gv.image_bytes,
                                s->stream, s->gv, a, st);
+            t->stop();
             in = st.out;
         }
+        Timer* t = start_timer(s, "clear_frontier");
         hipLaunchKernelGGL(k_clear_frontier, dim3(1024), dim3(kBlock), 0, s->stream, a, s->W);
+        t->stop();
         return 0;
     }
 };
@@ -1657,19 +1693,18 @@ int do_loop(cr_solver* s) {
     // kernels return at once).
     while (!done) {
         std::vector<std::unique_ptr<Timer>> timers;
+        s->timers = &timers;
         for (int l = 0; l < s->levels_per_sync; ++l) {
-            timers.emplace_back(new Timer(s, "cop_push"));
+            Timer* t = start_timer(s, "cop_push");
             if ((rc = dispatch_k<LaunchPush>(s->k, s, a)) != 0) return rc;
-            timers.back()->stop();
-            timers.emplace_back(new Timer(s, "cop_pull"));
+            t->stop();
             if (s->d_sep[0] && (rc = dispatch_k<LaunchPull>(s->k, s, a)) != 0) return rc;
-            timers.back()->stop();
-            timers.emplace_back(new Timer(s, "robber_step"));
+            t = start_timer(s, "robber_step");
             if ((rc = dispatch_w<LaunchRobber>(s->W, s, a)) != 0) return rc;
-            timers.back()->stop();
-            timers.emplace_back(new Timer(s, "cop_finalize"));
+            t->stop();
+            t = start_timer(s, "cop_finalize");
             if ((rc = dispatch_w<LaunchFinalize>(s->W, s, a)) != 0) return rc;
-            timers.back()->stop();
+            t->stop();
             hipLaunchKernelGGL(k_advance, dim3(1), dim3(kScanThreads), 0, s->stream, a);
             hipLaunchKernelGGL(k_write_lists, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, a, 0);
         }
@@ -1677,6 +1712,7 @@ int do_loop(cr_solver* s) {
         HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl, offsetof(Ctl, lvl), hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
         for (auto& t : timers) t->collect();
+        s->timers = nullptr;
         done = s->h_ctl->done != 0;
     }
     HIP_TRY(hipEventRecord(e1, s->stream));
@@ -1707,8 +1743,9 @@ int count_moves(cr_solver* s, int32_t mode, unsigned long long* r, MoveArgs mv =
     const uint64_t lds = ((s->gv.image_bytes + 15) & ~15u) + static_cast<uint64_t>(slots) * 8;
     if (lds > 160 * 1024)
         return fail(CR_ERR_UNSUPPORTED, "move count needs %llu bytes of LDS", static_cast<unsigned long long>(lds));
-    unsigned long long zero[3] = {0, 0, 0};
-    HIP_TRY(hipMemcpyAsync(s->d_ctl->retro, zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
+    static const unsigned long long zero[256] = {};
+    HIP_TRY(hipMemcpyAsync(s->d_ctl->retro, zero, 3 * sizeof(unsigned long long), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_ctl->hist, zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
     LevelArgs a = level_args(s);
     if (int rc = dispatch_k<LaunchRetro>(s->k, s, a, slots, mode, mv)) return rc;
     HIP_TRY(hipGetLastError());
@@ -1979,6 +2016,22 @@ int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_
     if (e_cop) *e_cop = r[0];
     if (e_rob) *e_rob = r[1];
     if (won) *won = r[2];
+    return 0;
+}
+
+int cr_work_profile(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint8_t* pulled) {
+    if (int rc = check_solver(s)) return rc;
+    if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    unsigned long long r[3];
+    if (int rc = count_moves(s, 0, r)) return rc;
+    Ctl* h = s->h_ctl;
+    HIP_TRY(hipMemcpyAsync(h, s->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int d = 0; d < 127; ++d) {
+        if (e_cop) e_cop[d] = h->hist[0][d];
+        if (e_rob) e_rob[d] = h->hist[1][d];
+        if (pulled) pulled[d] = static_cast<uint8_t>(h->dir[d] != 0);
+    }
     return 0;
 }
 

@@ -144,6 +144,12 @@ int cr_level_counts(cr_solver* s, uint64_t* out);
  * e_cop = sum over won robber-turn states (depth <= 125) of |pred(C')|,
  * e_rob = sum over won cop-turn states (depth <= 125) of in-degree(v). */
 int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_t* won);
+/* Retro-edges of the solved p = 1 table by depth (arrays of 127): e_cop[d]
+ * from won robber-turn states of depth d, e_rob[d] from won cop-turn states
+ * of depth d; pulled[d] = 1 when the cop turn of level d gathered instead of
+ * pushing (depth-0 relaxations are resolved in closed form by
+ * cr_initialize_captures). */
+int cr_work_profile(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint8_t* pulled);
 /* Total team moves sum over C of |T(C)|: the "Total edge pointers" the
  * reference prints after AuxGraph::createTransitions (AuxGraph.h:274). */
 int cr_count_transitions(cr_solver* s, uint64_t* total);

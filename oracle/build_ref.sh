@@ -28,4 +28,16 @@ done
 "$CXX" -std=c++17 -include cstddef -O3 -pthread -I"$REF/legacy/include" \
     "$REF/legacy/src/k_cops_4.cpp" "$REF/legacy/src/Graph.cpp" \
     "$REF/legacy/src/AdjacencyList.cpp" "$REF/legacy/src/fileio.cpp" -o "$OUT/k_cops_4"
+# The reference's own supervisor (app/main.cpp) with this package's plug-in
+# (copsandrobbers_amd/csrc/plugin_solver.cpp) as the solver translation unit:
+# the drop-in as a maintainer of the reference would build it, linked against
+# libcrsolve.so. Built only when the library is already there.
+REPO=$(cd "$HERE/.." && pwd)
+if [ -f "$REPO/copsandrobbers_amd/libcrsolve.so" ]; then
+    "$CXX" -std=c++17 -include cstddef -O3 -I"$REF/lib/include" -I"$REPO/include" \
+        "$REF/app/main.cpp" "$REF"/lib/src/*.cpp \
+        -DALGORITHM_INCLUDE="\"$REPO/copsandrobbers_amd/csrc/plugin_solver.cpp\"" \
+        -L"$REPO/copsandrobbers_amd" -lcrsolve -Wl,-rpath,'$ORIGIN/../../copsandrobbers_amd' \
+        -o "$OUT/k_cops_visibility_dropin"
+fi
 echo "build_ref: built $(ls "$OUT" | tr '\n' ' ')"

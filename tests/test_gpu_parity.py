@@ -141,6 +141,22 @@ def test_cli_reproduces_recorded_rows(tmp_path):
             assert str(res[col]) == row[col], (row["graph"], col)
 
 
+DROPIN = os.path.join(REPO, "oracle", "_ref", "k_cops_visibility_dropin")
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN), reason="drop-in not built (needs the reference tree at build time)")
+def test_dropin_in_reference_supervisor_reproduces_recorded_rows(tmp_path):
+    """The reference's own app/main.cpp compiled with the plug-in as its solver
+    translation unit (oracle/build_ref.sh) gives the recorded CSV facts."""
+    from copsandrobbers_amd import harness
+    for row in ROWS[1::4]:
+        say("dropin", row["graph"], row["k"])
+        res = harness.run_one(DROPIN, matrix(row["graph"]), int(row["k"]), 1, timeout=120, cwd=str(tmp_path))
+        assert res["status"] == "ok", res
+        for col in ("result", "capture_plys", "passes", "base_caps", "positions"):
+            assert str(res[col]) == row[col], (row["graph"], col)
+
+
 def test_error_paths(solver, tmp_path):
     say("errors")
     _, adj = cra.parse_matrix(open(matrix("peterson"), "rb").read())
