@@ -36,7 +36,8 @@ EXPORTS = (
     "cr_load_graph_file", "cr_load_graph_matrix", "cr_set_variant", "cr_build_aux_graph",
     "cr_initialize_captures", "cr_main_loop", "cr_find_final_result", "cr_save_aux_graph",
     "cr_get_info", "cr_get_stats", "cr_copy_states", "cr_copy_configs", "cr_level_counts",
-    "cr_count_retro_edges", "cr_work_profile", "cr_count_transitions", "cr_set_tuning", "cr_set_kernel_timing", "cr_kernel_times", "cr_cache_key",
+    "cr_count_retro_edges", "cr_work_profile", "cr_get_macro_stats", "cr_count_transitions",
+    "cr_set_tuning", "cr_set_kernel_timing", "cr_kernel_times", "cr_cache_key",
 )
 
 
@@ -63,6 +64,12 @@ class Stats(ctypes.Structure):
                 ("n_passes", ctypes.c_int32), ("n_levels", ctypes.c_int32),
                 ("init_ms", ctypes.c_double), ("loop_ms", ctypes.c_double),
                 ("pred_entries", ctypes.c_uint64), ("build_ms", ctypes.c_double)]
+
+
+class MacroStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int32), ("dag_nodes", ctypes.c_uint64), ("dag_edges", ctypes.c_uint64),
+                ("build_moves", ctypes.c_uint64), ("leaf_moves", ctypes.c_uint64),
+                ("up_edges", ctypes.c_uint64), ("build_ms", ctypes.c_double), ("passes_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -106,6 +113,7 @@ def lib():
         L.cr_count_retro_edges.argtypes = [vp, u64p, u64p, u64p]
         L.cr_count_transitions.argtypes = [vp, u64p]
         L.cr_work_profile.argtypes = [vp, u64p, u64p, u8p]
+        L.cr_get_macro_stats.argtypes = [vp, ctypes.POINTER(MacroStats), u64p, u64p, i32]
         L.cr_set_tuning.argtypes = [vp, i32, ctypes.c_int64]
         L.cr_set_kernel_timing.argtypes = [vp, i32]
         L.cr_kernel_times.argtypes = [vp, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
@@ -253,6 +261,19 @@ class Solver:
         pl = np.zeros(MAX_ROUND, np.uint8)
         _check(lib().cr_work_profile(self._h, _u64(ec), _u64(er), _u8(pl)))
         return ec, er, pl
+
+    def macro_stats(self):
+        """Trajectory DAG and work of the last limited-visibility main loop:
+        dict with steps, dag_nodes, dag_edges, build_moves, leaf_moves,
+        up_edges, build_ms, passes_ms, nodes_per_step, edges_per_step."""
+        m = MacroStats()
+        nodes = np.zeros(128, np.uint64)
+        edges = np.zeros(128, np.uint64)
+        _check(lib().cr_get_macro_stats(self._h, ctypes.byref(m), _u64(nodes), _u64(edges), 128))
+        out = {name: getattr(m, name) for name, _ in MacroStats._fields_}
+        out["nodes_per_step"] = [int(x) for x in nodes[:m.steps]]
+        out["edges_per_step"] = [int(x) for x in edges[:max(m.steps - 1, 0)]]
+        return out
 
     def count_transitions(self):
         t = ctypes.c_uint64()

@@ -46,19 +46,12 @@
 #include <string>
 #include <vector>
 
+#include "cr_device.h"
 #include "cr_host.h"
+#include "cr_macro.h"
 #include "crsolve.h"
 
-namespace {
-
-struct Timer;  // per-launch HIP-event pair (host side)
-
-constexpr int kWave = 64;
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr unsigned kMaxRound = CR_MAX_ROUND;
-constexpr uint8_t kUnmarked = static_cast<uint8_t>(kMaxRound << 1);
-
+namespace crd {
 thread_local char g_err[512] = "";
 
 int fail(int code, const char* fmt, ...) {
@@ -68,28 +61,11 @@ int fail(int code, const char* fmt, ...) {
     va_end(ap);
     return code;
 }
+}  // namespace crd
 
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail(CR_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));     \
-    } while (0)
+namespace {
 
-// Graph image copied into LDS by every block that ranks or unranks tuples:
-//   rank_tab u32[K*n]   rank_tab[i*n + x] = C(M-1-(x+i), K-i), M = n+K-1
-//   binom    u32[(K+1)*(M+1)]  binom[j*(M+1) + x] = C(x, j), saturated
-//   in_off   u16[n+1]   in-neighbour offsets
-//   in_nbr   u8[E]      in-neighbours (cop predecessors step back along them)
-//   out_off  u16[n+1], out_nbr u8[E]: out-neighbours (forward team moves)
-struct GraphView {
-    const uint8_t* image;
-    uint32_t image_bytes;
-    uint32_t off_binom, off_in_off, off_in_nbr, off_out_off, off_out_nbr;
-    int32_t n, W, M;
-    uint32_t C;
-    uint32_t ctot_m1;  // C - 1
-};
+struct Timer;  // per-launch HIP-event pair (host side)
 
 struct Ctl {
     uint32_t level;    // frontier depth d of the level being processed
@@ -156,117 +132,6 @@ __device__ __forceinline__ XcdSplit xcd_split(uint32_t n) {
 // Whether the level starting with `frontier_rows` robber-frontier rows pulls.
 __device__ __forceinline__ uint32_t choose_pull(uint32_t frontier_rows, uint32_t threshold) {
     return frontier_rows > threshold ? 1u : 0u;
-}
-
-// Copies the robber out-masks (n rows of W words) into LDS; returns the base.
-template <int W>
-__device__ __forceinline__ const uint64_t* stage_out_masks(uint64_t* lds_om, const uint64_t* out_mask, int n) {
-    for (int i = threadIdx.x; i < n * W; i += blockDim.x) lds_om[i] = out_mask[(i / W) * 4 + (i % W)];
-    __syncthreads();
-    return lds_om;
-}
-
-__device__ __forceinline__ uint32_t wave_id() {
-    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)));
-}
-
-__device__ __forceinline__ uint8_t won_byte(uint32_t d) { return static_cast<uint8_t>(1u | (d << 1)); }
-
-__device__ __forceinline__ void load_image(uint8_t* lds, const uint8_t* src, uint32_t bytes) {
-    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
-    uint32_t* d = reinterpret_cast<uint32_t*>(lds);
-    for (uint32_t i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
-    __syncthreads();
-}
-
-// Combinatorial-number-system decode of a lexicographic rank.
-template <int K>
-__device__ __forceinline__ void unrank(uint32_t rank, const GraphView& gv, const uint32_t* binom,
-                                       uint32_t* cops) {
-    uint32_t s = gv.ctot_m1 - rank;
-    int xmax = gv.M - 1;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-        const uint32_t* row = binom + (K - i) * (gv.M + 1);
-        int lo = 0, hi = xmax;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (row[mid] <= s) lo = mid; else hi = mid - 1;
-        }
-        s -= row[lo];
-        cops[i] = static_cast<uint32_t>(gv.M - 1 - lo - i);
-        xmax = lo - 1;
-    }
-}
-
-template <int K>
-__device__ __forceinline__ void sort_tuple(uint32_t* t) {
-#pragma unroll
-    for (int i = 0; i < K; ++i)
-#pragma unroll
-        for (int j = 0; j + 1 < K - i; ++j) {
-            const uint32_t a = t[j], b = t[j + 1];
-            t[j] = min(a, b);
-            t[j + 1] = max(a, b);
-        }
-}
-
-template <int K>
-__device__ __forceinline__ uint32_t rank_sorted(const uint32_t* t, const uint32_t* rank_tab, int n,
-                                                uint32_t ctot_m1) {
-    uint32_t s = 0;
-#pragma unroll
-    for (int i = 0; i < K; ++i) s += rank_tab[i * n + t[i]];
-    return ctot_m1 - s;
-}
-
-// Decodes product index t into one predecessor tuple (sorted) and returns
-// false when the tuple is a duplicate produced by swapping two stacked cops.
-template <int K>
-__device__ __forceinline__ bool product_tuple(uint32_t t, const uint32_t* cops, const uint32_t* deg,
-                                              const uint32_t* base, const uint8_t* nbr,
-                                              uint32_t* tup) {
-    uint32_t idx[K];
-#pragma unroll
-    for (int i = K - 1; i > 0; --i) {
-        const uint32_t q = t / deg[i];
-        idx[i] = t - q * deg[i];
-        t = q;
-    }
-    idx[0] = t;
-    bool keep = true;
-#pragma unroll
-    for (int i = 0; i + 1 < K; ++i)
-        if (cops[i] == cops[i + 1] && idx[i] > idx[i + 1]) keep = false;
-#pragma unroll
-    for (int i = 0; i < K; ++i) tup[i] = nbr[base[i] + idx[i]];
-    sort_tuple<K>(tup);
-    return keep;
-}
-
-
-// Block-wide sums of wave-uniform per-wave values; thread 0 gets the totals.
-// Every thread of the block must call it.
-__device__ __forceinline__ void block_sum2(unsigned long long& x, unsigned long long& y) {
-    __shared__ unsigned long long red[2][kWavesPerBlock];
-    __syncthreads();  // previous use of red is finished
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = x;
-        red[1][threadIdx.x >> 6] = y;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        x = y = 0;
-        for (int i = 0; i < kWavesPerBlock; ++i) {
-            x += red[0][i];
-            y += red[1][i];
-        }
-    }
-}
-
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -706,7 +571,6 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
 
 // Block-wide exclusive scan of gcnt[0, ng) in place; returns the total and the
 // sum of gstates[0, ng) to thread 0 (one workgroup of kScanThreads threads).
-constexpr int kScanThreads = 1024;
 
 __device__ void scan_groups(uint32_t* gcnt, const unsigned long long* gstates, uint32_t ng,
                             uint32_t& total, unsigned long long& states) {
@@ -840,71 +704,6 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
             const uint32_t idx = word * 64 + static_cast<uint32_t>(__builtin_ctzll(rest));
             rest &= rest - 1;
             dst[pos++] = (space1 && !init_mode) ? a.list0[prev][idx] : idx;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// exclusive scan of per-configuration counts into 64-bit CSR heads
-// ---------------------------------------------------------------------------
-constexpr int kScanTile = kScanThreads * 8;
-
-__device__ unsigned long long block_exclusive_scan(unsigned long long v, unsigned long long& total) {
-    __shared__ unsigned long long sh[kScanThreads];
-    const uint32_t t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
-        const unsigned long long x = t >= off ? sh[t - off] : 0ull;
-        __syncthreads();
-        sh[t] += x;
-        __syncthreads();
-    }
-    total = sh[kScanThreads - 1];
-    const unsigned long long r = sh[t] - v;
-    __syncthreads();
-    return r;
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_tile_sums(const uint32_t* cnt, uint64_t n,
-                                                            unsigned long long* tiles) {
-    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * 8ull;
-    unsigned long long v = 0;
-    for (int i = 0; i < 8; ++i)
-        if (base + i < n) v += cnt[base + i];
-    unsigned long long total;
-    block_exclusive_scan(v, total);
-    if (threadIdx.x == 0) tiles[blockIdx.x] = total;
-}
-
-// Scans the tile sums in place and stores the grand total as heads[n].
-__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(unsigned long long* tiles, uint32_t ntiles,
-                                                             unsigned long long* heads, uint64_t n) {
-    unsigned long long carry = 0;
-    for (uint32_t b0 = 0; b0 < ntiles; b0 += kScanThreads) {
-        const uint32_t i = b0 + threadIdx.x;
-        const unsigned long long v = i < ntiles ? tiles[i] : 0ull;
-        unsigned long long total;
-        const unsigned long long ex = block_exclusive_scan(v, total);
-        if (i < ntiles) tiles[i] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0) heads[n] = carry;
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_tile_apply(const uint32_t* cnt, uint64_t n,
-                                                             const unsigned long long* tiles,
-                                                             unsigned long long* heads) {
-    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * 8ull;
-    unsigned long long v = 0;
-    for (int i = 0; i < 8; ++i)
-        if (base + i < n) v += cnt[base + i];
-    unsigned long long total;
-    unsigned long long run = tiles[blockIdx.x] + block_exclusive_scan(v, total);
-    for (int i = 0; i < 8; ++i) {
-        if (base + i < n) {
-            heads[base + i] = run;
-            run += cnt[base + i];
         }
     }
 }
@@ -1117,20 +916,6 @@ struct KTime {
 
 enum Stage { kEmpty = 0, kLoaded, kBuilt, kInitialized, kSolved };
 
-template <typename T>
-T* dev_alloc(uint64_t count, uint64_t& total, int& rc) {
-    void* p = nullptr;
-    if (count == 0) count = 1;
-    const hipError_t e = hipMalloc(&p, count * sizeof(T));
-    if (e != hipSuccess) {
-        rc = fail(CR_ERR_DEVICE, "hipMalloc(%llu bytes): %s",
-                  static_cast<unsigned long long>(count * sizeof(T)), hipGetErrorString(e));
-        return nullptr;
-    }
-    total += count * sizeof(T);
-    return static_cast<T*>(p);
-}
-
 }  // namespace
 
 struct cr_solver {
@@ -1173,6 +958,10 @@ struct cr_solver {
     uint32_t push_blocks = 2048;
     uint32_t row_blocks = 2048;
     int32_t levels_per_sync = 8;
+    // limited-visibility variant (macro.hip)
+    crd::MacroPlan* macro = nullptr;
+    uint64_t* d_occ = nullptr;  // cop mask per configuration
+    crd::MacroProfile mprof;
 
     uint32_t nbatch() const { return static_cast<uint32_t>((C + 63) / 64); }
     uint64_t ngroups() const { return (nbatch() + kGroupWords - 1) / kGroupWords; }
@@ -1219,6 +1008,10 @@ struct cr_solver {
         hipFree(d_sep[1]);
         d_sep[0] = d_sep[1] = nullptr;
         csr_entries = 0;
+        crd::macro_free(macro, device_bytes);
+        macro = nullptr;
+        hipFree(d_occ);
+        d_occ = nullptr;
         hipFree(d_ctl);
         hipFree(d_image);
         hipFree(d_outmask);
@@ -1310,14 +1103,6 @@ struct Timer {
 Timer* start_timer(cr_solver* s, const char* name) {
     s->timers->emplace_back(new Timer(s, name));
     return s->timers->back().get();
-}
-
-// Kernels whose dynamic LDS (the graph image) exceeds the 64 KiB default need
-// the per-function limit raised before launch.
-int raise_lds_limit(const void* fn, uint32_t bytes) {
-    if (bytes <= 64 * 1024) return 0;
-    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
-    return 0;
 }
 
 template <int K, int W>
@@ -1575,6 +1360,29 @@ int alloc_pull(cr_solver* s) {
     return 0;
 }
 
+// Limited-visibility variant: one column of states, the marked plane (won0)
+// and the cop masks; the trajectory DAG is built by the main loop.
+int build_macro(cr_solver* s) {
+    const int n = s->g.n;
+    s->columns = 1;
+    s->S = s->C * static_cast<uint64_t>(n);
+    s->W = (n + 63) / 64;
+    const uint64_t plane = s->C * s->W;
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    const uint64_t need = s->S + 2 * plane * 8 + (4 << 20);
+    if (need > freeb) return fail(CR_ERR_TOO_LARGE, "state space needs %llu bytes; %zu free", static_cast<unsigned long long>(need), freeb);
+    int rc = 0;
+    if (!(s->d_table = dev_alloc<uint8_t>(s->S, s->device_bytes, rc))) return rc;
+    if (!(s->d_won0 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
+    if (!(s->d_occ = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
+    if (!(s->d_ctl = dev_alloc<Ctl>(1, s->device_bytes, rc))) return rc;
+    if (!s->h_ctl) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_ctl), sizeof(Ctl), hipHostMallocDefault));
+    if ((rc = upload_graph(s)) != 0) return rc;
+    s->stage = kBuilt;
+    return 0;
+}
+
 int do_build(cr_solver* s) {
     if (s->stage < kLoaded) return fail(CR_ERR_ORDER, "buildAuxGraph before loadGraphFile");
     s->release();
@@ -1582,11 +1390,16 @@ int do_build(cr_solver* s) {
     s->C = cr::config_count(n, s->k);
     if (s->C == 0) return fail(CR_ERR_K_RANGE, "no configuration for k=%d", s->k);
     if (s->C >= 0xffffffffull) return fail(CR_ERR_TOO_LARGE, "%llu configurations exceed 32-bit ids", static_cast<unsigned long long>(s->C));
-    if (s->variant != CR_VARIANT_COLUMNS || s->p != 1)
-        return fail(CR_ERR_UNSUPPORTED, "device solver supports the plain game (column solver, p = 1) only");
-    uint64_t prod = 1;
-    for (int i = 0; i < s->k; ++i) prod *= static_cast<uint64_t>(std::max(1, s->g.max_in));
-    if (prod >= (1ull << 32)) return fail(CR_ERR_TOO_LARGE, "team-move product %llu too large", static_cast<unsigned long long>(prod));
+    if (s->variant == CR_VARIANT_COLUMNS && s->p != 1)
+        return fail(CR_ERR_UNSUPPORTED, "the column solver runs on the device for p = 1 only");
+    uint64_t prod = 1, prod_out = 1;
+    for (int i = 0; i < s->k; ++i) {
+        prod *= static_cast<uint64_t>(std::max(1, s->g.max_in));
+        prod_out *= static_cast<uint64_t>(std::max(1, s->g.max_out));
+    }
+    if (prod >= (1ull << 32) || prod_out >= (1ull << 32))
+        return fail(CR_ERR_TOO_LARGE, "team-move product %llu too large", static_cast<unsigned long long>(std::max(prod, prod_out)));
+    if (s->variant == CR_VARIANT_MACRO) return build_macro(s);
     s->columns = 2 * s->p;
     s->S = s->C * static_cast<uint64_t>(n) * s->columns;
     s->W = (n + 63) / 64;
@@ -1628,10 +1441,53 @@ int reset_ctl(cr_solver* s) {
     return 0;
 }
 
+void add_ktime(void* ctx, const char* name, double ms) {
+    cr_solver* s = static_cast<cr_solver*>(ctx);
+    auto it = s->ktimes.find(name);
+    if (it == s->ktimes.end()) {
+        s->korder.push_back(name);
+        it = s->ktimes.emplace(name, KTime{}).first;
+    }
+    it->second.ms += ms;
+    it->second.launches += 1;
+}
+
+crd::TimeSink time_sink(cr_solver* s) {
+    crd::TimeSink t;
+    if (s->timing) {
+        t.ctx = s;
+        t.add = add_ktime;
+    }
+    return t;
+}
+
+int init_macro(cr_solver* s, uint64_t* base) {
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    uint64_t caps = 0;
+    int rc = crd::macro_init(s->gv, s->k, s->stream, s->d_table, s->d_occ, &caps);
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    s->stats = cr_stats{};
+    s->stats.init_ms = ms;
+    s->stats.base_captures = caps;
+    if (base) *base = caps;
+    s->stage = kInitialized;
+    return 0;
+}
+
 int do_init(cr_solver* s, uint64_t* base) {
     if (s->stage < kBuilt) return fail(CR_ERR_ORDER, "initializeCaptures before buildAuxGraph");
     int rc = reset_ctl(s);
     if (rc) return rc;
+    if (s->variant == CR_VARIANT_MACRO) return init_macro(s, base);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1676,10 +1532,25 @@ void passes_from_levels(cr_solver* s) {
     }
 }
 
+int loop_macro(cr_solver* s) {
+    int rc = crd::macro_solve(&s->macro, s->gv, s->d_outmask, s->k, s->p, s->stream, s->d_table, s->d_occ, s->d_won0,
+                              s->device_bytes, time_sink(s), s->pass_counts, s->mprof);
+    if (rc) return rc;
+    s->stats.loop_ms = s->mprof.build_ms + s->mprof.passes_ms;
+    s->stats.n_passes = static_cast<int32_t>(s->pass_counts.size());
+    s->stats.n_levels = s->stats.n_passes;
+    uint64_t won = s->stats.base_captures;
+    for (uint64_t c : s->pass_counts) won += c;
+    s->stats.won_states = won;
+    s->stage = kSolved;
+    return 0;
+}
+
 int do_loop(cr_solver* s) {
     if (s->stage < kInitialized) return fail(CR_ERR_ORDER, "mainLoop before initializeCaptures");
     s->ktimes.clear();
     s->korder.clear();
+    if (s->variant == CR_VARIANT_MACRO) return loop_macro(s);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1999,6 +1870,7 @@ int cr_copy_configs(cr_solver* s, uint8_t* dst, uint64_t first, uint64_t count) 
 
 int cr_level_counts(cr_solver* s, uint64_t* out) {
     if (int rc = check_solver(s)) return rc;
+    if (s->variant == CR_VARIANT_MACRO) return fail(CR_ERR_UNSUPPORTED, "depth levels belong to the column solver");
     if (out == nullptr) return fail(CR_ERR_ARG, "null output");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
     for (int d = 0; d < 127; ++d) {
@@ -2010,6 +1882,7 @@ int cr_level_counts(cr_solver* s, uint64_t* out) {
 
 int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_t* won) {
     if (int rc = check_solver(s)) return rc;
+    if (s->variant == CR_VARIANT_MACRO) return fail(CR_ERR_UNSUPPORTED, "depth levels belong to the column solver");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
     unsigned long long r[3];
     if (int rc = count_moves(s, 0, r)) return rc;
@@ -2021,6 +1894,7 @@ int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_
 
 int cr_work_profile(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint8_t* pulled) {
     if (int rc = check_solver(s)) return rc;
+    if (s->variant == CR_VARIANT_MACRO) return fail(CR_ERR_UNSUPPORTED, "depth levels belong to the column solver");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
     unsigned long long r[3];
     if (int rc = count_moves(s, 0, r)) return rc;
@@ -2031,6 +1905,29 @@ int cr_work_profile(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint8_t* pul
         if (e_cop) e_cop[d] = h->hist[0][d];
         if (e_rob) e_rob[d] = h->hist[1][d];
         if (pulled) pulled[d] = static_cast<uint8_t>(h->dir[d] != 0);
+    }
+    return 0;
+}
+
+int cr_get_macro_stats(cr_solver* s, cr_macro_stats* out, uint64_t* nodes_per_step, uint64_t* edges_per_step,
+                       int32_t cap) {
+    if (int rc = check_solver(s)) return rc;
+    if (out == nullptr) return fail(CR_ERR_ARG, "null output");
+    if (s->variant != CR_VARIANT_MACRO) return fail(CR_ERR_UNSUPPORTED, "no trajectory DAG in the column solver");
+    if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    const crd::MacroProfile& m = s->mprof;
+    std::memset(out, 0, sizeof *out);
+    out->steps = static_cast<int32_t>(m.nodes.size());
+    for (uint64_t x : m.nodes) out->dag_nodes += x;
+    for (uint64_t x : m.edges) out->dag_edges += x;
+    out->build_moves = m.build_moves;
+    out->leaf_moves = m.leaf_moves;
+    out->up_edges = m.up_edges;
+    out->build_ms = m.build_ms;
+    out->passes_ms = m.passes_ms;
+    for (int32_t d = 0; d < cap && d < out->steps; ++d) {
+        if (nodes_per_step) nodes_per_step[d] = m.nodes[d];
+        if (edges_per_step) edges_per_step[d] = d < static_cast<int32_t>(m.edges.size()) ? m.edges[d] : 0;
     }
     return 0;
 }

@@ -130,6 +130,7 @@ int cr_find_final_result(cr_solver* s, cr_verdict* out);
  * states to path. */
 int cr_save_aux_graph(cr_solver* s, const char* path);
 
+
 /* ---- accessors ------------------------------------------------------- */
 int cr_get_info(const cr_solver* s, cr_info* out);
 int cr_get_stats(const cr_solver* s, cr_stats* out);
@@ -150,6 +151,21 @@ int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_
  * pushing (depth-0 relaxations are resolved in closed form by
  * cr_initialize_captures). */
 int cr_work_profile(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint8_t* pulled);
+/* Limited-visibility variant (CR_VARIANT_MACRO): the trajectory DAG of the last
+ * mainLoop and the work it did. No counterpart in the reference, whose
+ * trajectory search (k_cops_visibility_v2.cpp:243-341) keeps no such graph. */
+typedef struct {
+    int32_t steps;        /* p: cop steps per macro turn */
+    uint64_t dag_nodes;   /* (configuration, cloud) nodes over all steps; step 1 = the states */
+    uint64_t dag_edges;   /* edges between consecutive steps */
+    uint64_t build_moves; /* team moves enumerated while building the DAG */
+    uint64_t leaf_moves;  /* (node, team move) pairs evaluated at step p, summed over passes */
+    uint64_t up_edges;    /* DAG edges read on the way up, summed over passes */
+    double build_ms;      /* device time of the DAG build */
+    double passes_ms;     /* device time of the passes */
+} cr_macro_stats;
+int cr_get_macro_stats(cr_solver* s, cr_macro_stats* out, uint64_t* nodes_per_step, uint64_t* edges_per_step,
+                       int32_t cap);
 /* Total team moves sum over C of |T(C)|: the "Total edge pointers" the
  * reference prints after AuxGraph::createTransitions (AuxGraph.h:274). */
 int cr_count_transitions(cr_solver* s, uint64_t* total);
