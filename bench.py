@@ -48,9 +48,17 @@ WORKLOADS = {
     "planar200_k4": ("planar200_e400_s1", 4),
     "sy_all_k2": ("scotlandyard-all", 2),
 }
+# limited-visibility workloads: (graph, k, p)
+MACRO_WORKLOADS = {
+    "grid8x8_k2_p3": ("grid8x8_chords16_s1", 2, 3),
+    "grid8x8_k2_p2": ("grid8x8_chords16_s1", 2, 2),
+}
 # solver timer name -> device kernel name in rocprofv3 summaries
 KERNEL_NAMES = {"cop_push": "k_cop_push", "sep_stage": "k_sep_stage", "robber_step": "k_robber_step",
-                "cop_finalize": "k_cop_finalize", "clear_frontier": "k_clear_frontier"}
+                "cop_finalize": "k_cop_finalize", "clear_frontier": "k_clear_frontier",
+                "mx_expand": "k_mx_expand", "mx_emit": "k_mx_expand", "mx_insert": "k_mx_insert",
+                "mx_compact": "k_mx_compact", "mx_leaf": "k_mx_leaf", "mx_up": "k_mx_up",
+                "mx_planes": "k_mx_planes", "mx_mark": "k_mx_mark"}
 
 
 def measured_traffic(workload, timer_name):
@@ -113,12 +121,120 @@ def copy_calibration(nbytes):
     return 0
 
 
+def cpu_reference_macro(stem, k, p, units):
+    """Reference v2 binary (single thread); rate in this solver's units of the same solve."""
+    exe = os.path.join(REPO, "oracle", "_ref", "k_cops_visibility_v2")
+    if not os.path.exists(exe):
+        return None
+    work = tempfile.mkdtemp(prefix="crbench_")
+    try:
+        proc = subprocess.run([exe, matrix_path(stem), str(k), str(p)], cwd=work, capture_output=True,
+                              text=True, timeout=600)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    m = re.search(r"Main Loop\s*-*=>\s*([\d.]+)\s*s", proc.stdout)
+    if not m:
+        return None
+    secs = float(m.group(1))
+    return {"value": units / secs / 1e9, "unit": "G macro-edges/s", "cores": 1, "kind": "reference",
+            "sample": f"reference k_cops_visibility_v2, same graph at p={p} ({stem}, k={k}), main loop "
+                      f"{secs:.2f} s; units = this solver's macro-edges for that solve ({units})"}
+
+
+def bench_macro(args):
+    """Limited-visibility solve (k_cops_visibility_v2.cpp). A step = initializeCaptures + mainLoop
+    (trajectory DAG build + all snapshot passes). Unit: a macro-edge, one (DAG node, team move)
+    evaluation -- enumerated while building the DAG, at the last cop step of every pass, or a DAG
+    edge read on the way up in every pass."""
+    import torch
+
+    import copsandrobbers_amd as cra
+    stem, k, p = MACRO_WORKLOADS[args.workload]
+    s = cra.Solver(0)
+    s.load_graph_file(matrix_path(stem), k, p, cra.VARIANT_MACRO)
+    s.build_aux_graph()
+    info = s.info()
+    for _ in range(args.warmup):
+        s.initialize_captures()
+        s.main_loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.initialize_captures()
+        passes = s.main_loop()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    m = s.macro_stats()
+    units = m["build_moves"] + m["leaf_moves"] + m["up_edges"]
+    # kernel split, from a separate timed run (events add host overhead)
+    s.set_kernel_timing(True)
+    s.initialize_captures()
+    s.main_loop()
+    kt = s.kernel_times()
+    s.set_kernel_timing(False)
+    verdict = s.find_final_result()
+    W = (info.n + 63) // 64
+    key = 4 + 8 * W  # configuration rank + cloud words of one DAG node
+    nodes = m["nodes_per_step"]
+    leaf_nodes, inner_nodes = nodes[-1], sum(nodes[:-1])
+    E = m["dag_edges"]
+    npass = len(passes)
+    alg = {  # scalar-record bytes per solve and kernel
+        "mx_expand": inner_nodes * (key + 5),          # node key read, value + count written
+        "mx_emit": inner_nodes * key + E * key,        # node key read, candidate key written
+        "mx_insert": E * (key + 8),                    # candidate key read, slot CAS, owner written
+        "mx_compact": E * 16 + sum(nodes[1:]) * key,   # owner + position read, edge written, node key copied
+        "mx_leaf": npass * leaf_nodes * (key + 1) + m["leaf_moves"] * info.n / 8,  # node + N-bit row per move
+        "mx_up": npass * (E * 5 + inner_nodes * 9),    # edge + child value read, offsets read, value written
+        "mx_planes": npass * info.num_states * 2,      # snapshot row byte read, plane bits written
+        "mx_mark": npass * info.num_states * 2,        # value read, table byte read (and written once)
+    }
+    dominant = max(kt, key=lambda n: kt[n][0])
+    rl = None
+    if dominant in alg:
+        ms, launches = kt[dominant]
+        per_launch = alg[dominant] / launches
+        achieved = per_launch / (ms / 1e3 / launches)
+        rl = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+              "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.workload, dominant),
+              "kernel": KERNEL_NAMES[dominant], "alg_bytes_per_launch": per_launch,
+              "avg_launch_us": ms * 1e3 / launches}
+    out = {
+        "metric": "macro-edges/s (limited visibility)", "value": units * args.steps / elapsed / 1e9,
+        "unit": "G macro-edges/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8/u64 bitsets (integer)",
+        "data": "synthetic (seeded graph generator, committed matrix)",
+        "config": {"workload": args.workload, "graph": stem, "n": info.n, "k": k, "p": p,
+                   "configs": info.config_count, "states": info.num_states, "passes": len(passes),
+                   "verdict": "WIN" if verdict else "LOSS", "dag_nodes_per_step": m["nodes_per_step"],
+                   "dag_edges_per_step": m["edges_per_step"], "macro_edges": units,
+                   "build_ms": m["build_ms"], "passes_ms": m["passes_ms"]},
+        "roofline": rl,
+        "kernels_ms_per_step": {n: v[0] for n, v in kt.items()},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        # bounded sample: the reference at p - 1 on the same graph (p = 3 takes ~2 min there)
+        sp = max(1, p - 1)
+        with cra.Solver(0) as t:
+            t.load_graph_file(matrix_path(stem), k, sp, cra.VARIANT_MACRO)
+            t.build_aux_graph()
+            t.initialize_captures()
+            t.main_loop()
+            tm = t.macro_stats()
+        out["cpu_baseline"] = cpu_reference_macro(stem, k, sp, tm["build_moves"] + tm["leaf_moves"] + tm["up_edges"])
+    s.close()
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="er200_k3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="er200_k3", choices=sorted(WORKLOADS) + sorted(MACRO_WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-port", type=int, default=-1,
                     help="threads for the oracle-port timing (default: min(16, cores); 0 = skip)")
@@ -137,6 +253,8 @@ def main():
 
     import copsandrobbers_amd as cra
 
+    if args.workload in MACRO_WORKLOADS:
+        return bench_macro(args)
     stem, k = WORKLOADS[args.workload]
     path = matrix_path(stem)
     s = cra.Solver(0)

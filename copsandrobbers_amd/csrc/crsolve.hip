@@ -979,6 +979,7 @@ struct cr_solver {
     std::vector<std::unique_ptr<Timer>>* timers = nullptr;  // events of the chunk in flight
     std::map<std::string, KTime> ktimes;
     std::vector<std::string> korder;
+    std::vector<hipEvent_t> events;  // reusable timing events
 
     ~cr_solver() { release(); }
 
@@ -1026,6 +1027,19 @@ struct cr_solver {
 
 namespace {
 
+// Timing events are created once and reused: creating a pair per launch costs
+// more host time than the short level kernels take on the device.
+hipEvent_t acquire_event(cr_solver* s) {
+    if (s->events.empty()) {
+        hipEvent_t e = nullptr;
+        hipEventCreate(&e);
+        return e;
+    }
+    hipEvent_t e = s->events.back();
+    s->events.pop_back();
+    return e;
+}
+
 int check_solver(const cr_solver* s) {
     if (s == nullptr) return fail(CR_ERR_ARG, "null solver");
     return 0;
@@ -1072,8 +1086,8 @@ struct Timer {
     hipEvent_t a = nullptr, b = nullptr;
     Timer(cr_solver* s_, const char* n) : s(s_), name(n) {
         if (s->timing) {
-            hipEventCreate(&a);
-            hipEventCreate(&b);
+            a = acquire_event(s);
+            b = acquire_event(s);
             hipEventRecord(a, s->stream);
         }
     }
@@ -1093,8 +1107,8 @@ struct Timer {
         }
         it->second.ms += ms;
         it->second.launches += 1;
-        hipEventDestroy(a);
-        hipEventDestroy(b);
+        s->events.push_back(a);
+        s->events.push_back(b);
         a = b = nullptr;
     }
 };
@@ -1720,6 +1734,7 @@ void cr_solver_destroy(cr_solver* s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     s->release();
     if (s->h_ctl) hipHostFree(s->h_ctl);
+    for (hipEvent_t e : s->events) hipEventDestroy(e);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
 }

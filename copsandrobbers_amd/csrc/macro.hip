@@ -565,16 +565,25 @@ int ensure(DevBuf& b, uint64_t count, uint64_t& device_bytes) {
 struct Spans {
     crd::TimeSink sink;
     hipStream_t st;
+    std::vector<hipEvent_t>* pool;  // reused across solves
     struct Span {
         const char* name;
         hipEvent_t a, b;
     };
     std::vector<Span> open;
+    hipEvent_t take() {
+        hipEvent_t e = nullptr;
+        if (pool->empty()) {
+            hipEventCreate(&e);
+        } else {
+            e = pool->back();
+            pool->pop_back();
+        }
+        return e;
+    }
     void begin(const char* name) {
         if (!sink.add) return;
-        Span s{name, nullptr, nullptr};
-        hipEventCreate(&s.a);
-        hipEventCreate(&s.b);
+        Span s{name, take(), take()};
         hipEventRecord(s.a, st);
         open.push_back(s);
     }
@@ -587,8 +596,8 @@ struct Spans {
             float ms = 0;
             hipEventElapsedTime(&ms, s.a, s.b);
             sink.add(sink.ctx, s.name, ms);
-            hipEventDestroy(s.a);
-            hipEventDestroy(s.b);
+            pool->push_back(s.a);
+            pool->push_back(s.b);
         }
         open.clear();
     }
@@ -606,6 +615,7 @@ struct MacroPlan {
     std::vector<Step> steps;  // steps[d-1] = cop step d
     DevBuf cnt, cand_cfg, cand_cloud, rep, flag, slots, tiles, pos, planes, counters;
     unsigned long long* h_counters = nullptr;  // pinned: found, leaf moves, build moves, scratch
+    std::vector<hipEvent_t> events;            // timing events, reused
 
     void release(uint64_t& device_bytes) {
         auto drop = [&](DevBuf& b) {
@@ -626,6 +636,8 @@ struct MacroPlan {
         for (DevBuf* b : {&cnt, &cand_cfg, &cand_cloud, &rep, &flag, &slots, &tiles, &pos, &planes, &counters}) drop(*b);
         if (h_counters) hipHostFree(h_counters);
         h_counters = nullptr;
+        for (hipEvent_t e : events) hipEventDestroy(e);
+        events.clear();
     }
 };
 
@@ -709,7 +721,7 @@ int macro_solve(MacroPlan** planp, const GraphView& gv, const uint64_t* d_nbr_ma
     prof.candidates.assign(p > 1 ? p - 1 : 0, 0);
     prof.edges.assign(p > 1 ? p - 1 : 0, 0);
     passes.clear();
-    Spans spans{sink, st, {}};
+    Spans spans{sink, st, &pl.events, {}};
 
     hipEvent_t e0, e1, e2;
     HIP_TRY(hipEventCreate(&e0));
