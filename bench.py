@@ -229,12 +229,148 @@ def bench_macro(args):
     return 0
 
 
+def bench_batch(args):
+    """4096 seeded connected graphs (n uniform in [8, 48], edge probability in
+    [0.08, 0.3], k in {1, 2, 3}), full information. A step = one
+    cr_batch_solve over all of them (one persistent launch; tables stay in
+    HBM, verdicts come back), timed on the host clock around the call. Beside
+    it: the plain six-step single-graph loop on a sample of the same graphs,
+    and a Poisson request stream through the continuous-batching server."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    import copsandrobbers_amd as cra
+    from copsandrobbers_amd import batch as crb
+    from copsandrobbers_amd import workloads
+
+    graphs = workloads.batch_graphs(4096)
+    states = sum(2 * __import__("math").comb(a.shape[0] + k - 1, k) * a.shape[0] for a, k in graphs)
+    b = crb.Batch(0)
+    for _ in range(args.warmup):
+        ref = b.solve(graphs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    for _ in range(args.steps):
+        ref = b.solve(graphs)
+        dev_ms += b.last_timing()[0]
+    elapsed = time.perf_counter() - t0
+    grid = b.last_timing()[1]
+    batched = len(graphs) * args.steps / elapsed
+    # the plain loop: the six steps per graph on an evenly spaced sample
+    idx = np.linspace(0, len(graphs) - 1, min(args.plain_sample, len(graphs))).astype(int)
+    s = cra.Solver(0)
+    s.load_graph_matrix(graphs[0][0], graphs[0][1], 1)
+    s.build_aux_graph()
+    t1 = time.perf_counter()
+    for i in idx:
+        adj, k = graphs[i]
+        s.load_graph_matrix(adj, k, 1)
+        s.build_aux_graph()
+        s.initialize_captures()
+        s.main_loop()
+        v = s.find_final_result()
+        assert (v is not None) == ref[i]["win"]
+    plain = len(idx) / (time.perf_counter() - t1)
+    s.close()
+    # serving: Poisson arrivals at a fraction of the batched rate
+    rate = args.serve_load * batched
+    rng = np.random.default_rng(12345)
+    nreq = len(graphs)
+    gaps = rng.exponential(1.0 / rate, nreq)
+    order = rng.permutation(len(graphs))[:nreq]
+    results = {}
+    with crb.Server(0, hbm_budget=64 << 30, max_batch=512, streams=4) as srv:
+        start = crb.clock_ns() + 2_000_000
+        arrivals = (start + (np.cumsum(gaps) * 1e9)).astype(np.uint64)
+        tickets = {}
+
+        def produce():
+            # hands every request whose arrival time has passed to the server in one call
+            j = 0
+            while j < nreq:
+                now = crb.clock_ns()
+                hi = int(np.searchsorted(arrivals, now, side="right"))
+                if hi <= j:
+                    continue
+                ts, _ = srv.submit_many([graphs[gi] for gi in order[j:hi]], arrivals[j:hi])
+                for t, gi in zip(ts, order[j:hi]):
+                    tickets[t] = gi
+                j = hi
+
+        th = threading.Thread(target=produce)
+        th.start()
+        while len(results) < nreq:
+            for r in srv.poll(wait_us=500):
+                results[r["ticket"]] = r
+        th.join()
+        t_end = crb.clock_ns()
+        sstats = srv.stats()
+    lat = np.array([r["latency_ms"] for r in results.values()])
+    mismatch = sum(1 for t, r in results.items() if r["table_hash"] != ref[tickets[t]]["table_hash"])
+    out = {
+        "metric": "batched graphs/s (4096 small graphs, one launch)", "value": batched, "unit": "graphs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8/u64 bitsets (integer)",
+        "data": "synthetic (seeded graph generator)",
+        "config": {"workload": "batch4096", "graphs": len(graphs), "states": states, "workgroups": grid,
+                   "device_ms_per_step": dev_ms / args.steps},
+        "plain_loop": {"graphs_per_s": plain, "sample": int(len(idx)), "speedup_batched": batched / plain},
+        "serving": {"arrival_rate_per_s": rate, "requests": nreq, "served_per_s": nreq / ((t_end - start) / 1e9),
+                    "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                    "launches": sstats["launches"], "peak_admitted_bytes": sstats["peak_bytes"],
+                    "results_differing_from_batch": mismatch},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_reference_batch(graphs, idx[:: max(1, len(idx) // 32)])
+    b.close()
+    print(json.dumps(out))
+    return 0
+
+
+def cpu_reference_batch(graphs, sample, budget_s=60.0):
+    """The reference binary, one process per graph as its harness runs it, on
+    a sample of the batch (stops after budget_s); graphs/s of the graphs done."""
+    exe = os.path.join(REPO, "oracle", "_ref", "k_cops_visibility")
+    if not os.path.exists(exe):
+        return None
+    from copsandrobbers_amd import workloads
+    work = tempfile.mkdtemp(prefix="crbatch_")
+    done, t_total = 0, 0.0
+    try:
+        for i in sample:
+            adj, k = graphs[i]
+            path = os.path.join(work, f"g{i}.txt")
+            workloads.write_matrix(path, adj)
+            t0 = time.perf_counter()
+            proc = subprocess.run([exe, path, str(k), "1"], cwd=work, capture_output=True, text=True, timeout=300)
+            t_total += time.perf_counter() - t0
+            if "RESULT" not in proc.stdout:
+                return None
+            done += 1
+            if t_total > budget_s:
+                break
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return {"value": done / t_total, "unit": "graphs/s", "cores": 1, "kind": "reference",
+            "sample": f"reference k_cops_visibility, one process per graph, {done} graphs evenly spaced "
+                      f"over the batch, {t_total:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="er200_k3", choices=sorted(WORKLOADS) + sorted(MACRO_WORKLOADS))
+    ap.add_argument("--workload", default="er200_k3",
+                    choices=sorted(WORKLOADS) + sorted(MACRO_WORKLOADS) + ["batch4096"])
+    ap.add_argument("--plain-sample", type=int, default=256,
+                    help="batch4096: graphs timed through the plain per-graph loop")
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="batch4096: Poisson arrival rate as a fraction of the batched throughput")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-port", type=int, default=-1,
                     help="threads for the oracle-port timing (default: min(16, cores); 0 = skip)")
@@ -255,6 +391,8 @@ def main():
 
     if args.workload in MACRO_WORKLOADS:
         return bench_macro(args)
+    if args.workload == "batch4096":
+        return bench_batch(args)
     stem, k = WORKLOADS[args.workload]
     path = matrix_path(stem)
     s = cra.Solver(0)

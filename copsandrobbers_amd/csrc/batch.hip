@@ -1,0 +1,848 @@
+// batch.hip -- many small plain games (p = 1) in one persistent launch, and a
+// continuous-batching server on top of it.
+//
+// Each workgroup takes graphs from a device work queue (largest first) and
+// solves one graph at a time with the whole workgroup, so a graph's depth
+// levels need only workgroup barriers and graphs never wait on each other.
+// Per graph (n <= 64 vertices: one 64-bit robber mask per configuration row),
+// in the workgroup's scratch slot:
+//   won0 / won1   states won with the cops / the robber to move
+//   fr1           robber-turn states of the current depth
+//   nx0 / nx1     new cop-turn / robber-turn states of the next depth
+//   H0 / H1       intermediates of the factorised cop-turn pull
+// Level d (states of depth d -> depth d+1; the level-synchronous form of
+// k_cops_visibility.cpp:129-299, as in the single-graph solver):
+//   cop turn     nx0[C] = OR_{C' in T(C)} fr1[C'] & ~won0[C], taken one cop
+//                at a time: H_i[A][B] = OR_{v in N(max A)} H_{i-1}[A - max A][B + v]
+//                (A: i cops still at their old vertices, B: K-i moved cops),
+//                H_0 = fr1, H_K = nx0;
+//   robber turn  nx1[C] = { r not won : every move of r lands in won0[C] };
+//   finalize     table bytes at depth d+1, won |= new, fr1 = nx1.
+// Levels start at d = 0 from the captures and stop at the first level that
+// wins nothing or before depth 127 (the 7-bit cap, :189, :280).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "cr_device.h"
+#include "cr_host.h"
+
+namespace {
+
+constexpr int kMaxBK = CR_BATCH_MAX_K;
+constexpr int kMaxBN = CR_BATCH_MAX_N;
+constexpr int kBinomCols = kMaxBN + kMaxBK;  // x in [0, n + K - 1]
+
+struct BatchGraph {
+    uint32_t n, k, C, hmax;   // hmax: largest intermediate H_i (rows)
+    uint64_t mask_off;        // n out-masks at masks + mask_off
+    uint64_t table_off;       // 2*C*n table bytes at tables + table_off
+};
+
+struct BatchOut {
+    unsigned long long verdict;  // worst << 32 | cId of the best universal row, ~0 if none
+    unsigned long long base, won, hash;
+    uint32_t levels, pad;
+    uint32_t hist[CR_MAX_ROUND][2];  // states won per depth and column
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+// Block-wide sums with the result in every thread.
+__device__ __forceinline__ void block_sum_all(unsigned long long& x, unsigned long long& y,
+                                              unsigned long long* red /* [2][kWavesPerBlock] */) {
+    x = wave_sum(x);
+    y = wave_sum(y);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = x;
+        red[kWavesPerBlock + (threadIdx.x >> 6)] = y;
+    }
+    __syncthreads();
+    x = y = 0;
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+        x += red[i];
+        y += red[kWavesPerBlock + i];
+    }
+}
+
+__device__ __forceinline__ unsigned long long block_min_all(unsigned long long x, unsigned long long* red) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(x, off);
+        x = o < x ? o : x;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    x = ~0ull;
+    for (int i = 0; i < kWavesPerBlock; ++i) x = red[i] < x ? red[i] : x;
+    return x;
+}
+
+// C(x, y) for the small arguments of one graph (exact: every prefix product
+// divides).
+__device__ __forceinline__ uint32_t small_binom(int x, int y) {
+    if (y < 0 || x < y) return 0;
+    uint64_t c = 1;
+    for (int i = 0; i < y; ++i) c = c * static_cast<uint64_t>(x - i) / static_cast<uint64_t>(i + 1);
+    return static_cast<uint32_t>(c);
+}
+
+template <int K>
+__device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ masks, uint8_t* __restrict__ tables,
+                            BatchOut* __restrict__ out, uint64_t* __restrict__ scratch, uint64_t* s_mask,
+                            uint32_t* s_binom, unsigned long long* s_red) {
+    const int n = static_cast<int>(g.n), Mimg = n + K - 1;
+    const uint32_t C = g.C, T = blockDim.x, tid = threadIdx.x;
+    const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
+    for (int i = tid; i < n; i += T) s_mask[i] = masks[g.mask_off + i];
+    for (int i = tid; i < (K + 1) * (Mimg + 1); i += T) s_binom[i] = small_binom(i % (Mimg + 1), i / (Mimg + 1));
+    __syncthreads();
+    uint32_t cnt[K + 1];
+#pragma unroll
+    for (int j = 0; j <= K; ++j) cnt[j] = s_binom[j * (Mimg + 1) + (n + j - 1)];  // C(n+j-1, j)
+    uint64_t* won0 = scratch;
+    uint64_t* won1 = won0 + C;
+    uint64_t* fr1 = won1 + C;
+    uint64_t* nx0 = fr1 + C;
+    uint64_t* nx1 = nx0 + C;
+    uint64_t* hbuf[2] = {nx1 + C, nx1 + C + g.hmax};
+    uint8_t* last0 = reinterpret_cast<uint8_t*>(nx1 + C + 2ull * g.hmax);
+    uint8_t* table = tables + g.table_off;
+
+    // captures: depth 0 in both columns (k_cops_visibility.cpp:85-127)
+    unsigned long long base = 0, unused = 0;
+    for (uint32_t c = tid; c < C; c += T) {
+        uint32_t cops[K];
+        unrank_j(c, K, n, s_binom, Mimg, cops);
+        uint64_t occ = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) occ |= 1ull << cops[j];
+        won0[c] = occ;
+        won1[c] = occ;
+        fr1[c] = occ;
+        last0[c] = 0;
+        uint8_t* row = table + static_cast<uint64_t>(c) * n * 2;
+        for (int r = 0; r < n; ++r) {
+            const uint8_t b = ((occ >> r) & 1) ? won_byte(0) : kUnmarked;
+            row[2 * r] = b;
+            row[2 * r + 1] = b;
+        }
+        base += __builtin_popcountll(occ);
+    }
+    block_sum_all(base, unused, s_red);
+    if (tid == 0) {
+        out->hist[0][0] = static_cast<uint32_t>(base);
+        out->hist[0][1] = static_cast<uint32_t>(base);
+        for (int d = 1; d < CR_MAX_ROUND; ++d) out->hist[d][0] = out->hist[d][1] = 0;
+    }
+    unsigned long long won = 2 * base, prev0 = base, prev1 = base;
+    uint32_t levels = 0;
+    for (uint32_t d = 0; d + 1 < CR_MAX_ROUND; ++d) {
+        const bool do_cop = prev1 != 0, do_rob = prev0 != 0;
+        // cop turn: factorised pull from the robber frontier
+        if (do_cop) {
+#pragma unroll
+            for (int i = 1; i <= K; ++i) {
+                const uint64_t* in = i == 1 ? fr1 : hbuf[(i - 2) & 1];
+                uint64_t* dst = i == K ? nx0 : hbuf[(i - 1) & 1];
+                const uint32_t cb = cnt[K - i], cb_in = cnt[K - i + 1], rows = cnt[i] * cb;
+                for (uint32_t o = tid; o < rows; o += T) {
+                    const uint32_t ra = o / cb, rb = o - ra * cb;
+                    uint32_t A[K], B[K + 1];
+                    unrank_j(ra, i, n, s_binom, Mimg, A);
+                    unrank_j(rb, K - i, n, s_binom, Mimg, B);
+                    const uint32_t ra_in = rank_j(A, i - 1, n, s_binom, Mimg);
+                    uint64_t acc = 0, bits = s_mask[A[i - 1]];
+                    while (bits) {
+                        const uint32_t v = static_cast<uint32_t>(__builtin_ctzll(bits));
+                        bits &= bits - 1;
+                        uint32_t M2[K + 1];
+                        int pos = 0;
+                        bool placed = false;
+                        for (int t = 0; t < K - i; ++t) {
+                            if (!placed && v <= B[t]) {
+                                M2[pos++] = v;
+                                placed = true;
+                            }
+                            M2[pos++] = B[t];
+                        }
+                        if (!placed) M2[pos++] = v;
+                        acc |= in[static_cast<uint64_t>(ra_in) * cb_in + rank_j(M2, K - i + 1, n, s_binom, Mimg)];
+                    }
+                    dst[o] = i == K ? (acc & ~won0[o] & full) : acc;
+                }
+                __syncthreads();
+            }
+        }
+        // robber turn: every move of r lands on a won cop-turn state (depth <= d)
+        if (do_rob) {
+            for (uint32_t c = tid; c < C; c += T) {
+                const uint64_t w0 = won0[c];
+                uint64_t cand = full & ~won1[c], fresh = 0;
+                while (cand) {
+                    const int r = __builtin_ctzll(cand);
+                    cand &= cand - 1;
+                    if ((s_mask[r] & ~w0) == 0) fresh |= 1ull << r;
+                }
+                nx1[c] = fresh;
+            }
+        }
+        __syncthreads();
+        // finalize: depth d+1
+        unsigned long long c0 = 0, c1 = 0;
+        const uint8_t byte = won_byte(d + 1);
+        for (uint32_t c = tid; c < C; c += T) {
+            const uint64_t a = do_cop ? nx0[c] : 0, b = do_rob ? nx1[c] : 0;
+            uint8_t* row = table + static_cast<uint64_t>(c) * n * 2;
+            if (a) {
+                won0[c] |= a;
+                last0[c] = static_cast<uint8_t>(d + 1);
+                uint64_t m = a;
+                while (m) {
+                    row[2 * __builtin_ctzll(m)] = byte;
+                    m &= m - 1;
+                }
+                c0 += __builtin_popcountll(a);
+            }
+            if (b) {
+                won1[c] |= b;
+                uint64_t m = b;
+                while (m) {
+                    row[2 * __builtin_ctzll(m) + 1] = byte;
+                    m &= m - 1;
+                }
+                c1 += __builtin_popcountll(b);
+            }
+            fr1[c] = b;
+        }
+        block_sum_all(c0, c1, s_red);
+        if (tid == 0) {
+            out->hist[d + 1][0] = static_cast<uint32_t>(c0);
+            out->hist[d + 1][1] = static_cast<uint32_t>(c1);
+        }
+        won += c0 + c1;
+        prev0 = c0;
+        prev1 = c1;
+        levels = d + 1;
+        if (c0 + c1 == 0) break;
+    }
+    // findFinalResult (k_cops_visibility.cpp:301-354): universal rows, least
+    // worst depth, lowest cId
+    unsigned long long best = ~0ull;
+    for (uint32_t c = tid; c < C; c += T)
+        if (won0[c] == full) {
+            const unsigned long long key = (static_cast<unsigned long long>(last0[c]) << 32) | c;
+            best = key < best ? key : best;
+        }
+    best = block_min_all(best, s_red);
+    // order-independent table hash
+    unsigned long long h = 0;
+    const uint64_t bytes = 2ull * C * n;
+    for (uint64_t i = tid; i < bytes; i += T) h += (static_cast<uint64_t>(table[i]) + 1) * mix64(i);
+    unused = 0;
+    block_sum_all(h, unused, s_red);
+    if (tid == 0) {
+        out->verdict = best;
+        out->base = base;
+        out->won = won;
+        out->hash = h;
+        out->levels = levels;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_solve(const BatchGraph* __restrict__ graphs,
+                                                        const uint32_t* __restrict__ order, uint32_t count,
+                                                        const uint64_t* __restrict__ masks, uint8_t* tables,
+                                                        BatchOut* out, uint64_t* scratch, uint64_t slot_words,
+                                                        unsigned int* next) {
+    __shared__ uint64_t s_mask[kMaxBN];
+    __shared__ uint32_t s_binom[(kMaxBK + 1) * (kBinomCols + 1)];
+    __shared__ unsigned long long s_red[2 * kWavesPerBlock];
+    __shared__ uint32_t s_item;
+    uint64_t* slot = scratch + static_cast<uint64_t>(blockIdx.x) * slot_words;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(next, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        __syncthreads();  // every thread has read s_item before it is rewritten
+        if (item >= count) break;
+        const uint32_t gi = order[item];
+        const BatchGraph g = graphs[gi];
+        switch (g.k) {
+            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red); break;
+            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red); break;
+            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red); break;
+            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red); break;
+            default: break;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+using Clock = std::chrono::steady_clock;
+
+uint64_t now_ns() {
+    return static_cast<uint64_t>(std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now().time_since_epoch()).count());
+}
+
+struct Buf {
+    void* p = nullptr;
+    uint64_t cap = 0;
+    bool pinned = false;
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+int grow(Buf& b, uint64_t bytes, bool pinned) {
+    bytes = std::max<uint64_t>(bytes, 64);
+    if (b.cap >= bytes) return 0;
+    if (b.p) {
+        if (b.pinned) hipHostFree(b.p);
+        else hipFree(b.p);
+    }
+    b.p = nullptr;
+    b.cap = 0;
+    b.pinned = pinned;
+    if (pinned) {
+        HIP_TRY(hipHostMalloc(&b.p, bytes, hipHostMallocDefault));
+    } else {
+        size_t freeb = 0, totalb = 0;
+        HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+        if (bytes + (512ull << 20) > freeb)
+            return fail(CR_ERR_TOO_LARGE, "batch needs %llu more device bytes; %zu free",
+                        static_cast<unsigned long long>(bytes), freeb);
+        HIP_TRY(hipMalloc(&b.p, bytes));
+    }
+    b.cap = bytes;
+    return 0;
+}
+
+void drop(Buf& b) {
+    if (b.p) {
+        if (b.pinned) hipHostFree(b.p);
+        else hipFree(b.p);
+    }
+    b = Buf{};
+}
+
+// One graph of a batch, validated and sized on the host.
+struct Item {
+    int32_t status = CR_OK;
+    uint32_t n = 0, k = 0, C = 0, hmax = 0;
+    uint64_t table_bytes = 0, slot_words = 0;
+    double cost = 0;
+    uint64_t masks[kMaxBN] = {};
+};
+
+// Sizes and validates one graph; returns its status (the message stays in
+// the thread's last error).
+int prepare(Item& it, int32_t n, int32_t k, const uint8_t* adj) {
+    if (n <= 0) return it.status = fail(CR_ERR_EMPTY_GRAPH, "graph is empty");
+    if (n > kMaxBN) return it.status = fail(CR_ERR_GRAPH_LIMIT, "batched graphs have at most %d vertices (got %d)", kMaxBN, n);
+    if (k <= 0 || k > kMaxBK) return it.status = fail(CR_ERR_K_RANGE, "batched graphs have 1..%d cops (got %d)", kMaxBK, k);
+    if (adj == nullptr) return it.status = fail(CR_ERR_ARG, "null matrix");
+    uint64_t edges = 0;
+    for (int u = 0; u < n; ++u) {
+        uint64_t m = 0;
+        for (int v = 0; v < n; ++v)
+            if (adj[static_cast<size_t>(u) * n + v]) m |= 1ull << v;
+        if (m == 0) return it.status = fail(CR_ERR_ISOLATED_VERTEX, "vertex %d has no move", u);
+        it.masks[u] = m;
+        edges += static_cast<uint64_t>(__builtin_popcountll(m));
+    }
+    it.n = static_cast<uint32_t>(n);
+    it.k = static_cast<uint32_t>(k);
+    it.C = static_cast<uint32_t>(cr::config_count(n, k));
+    uint64_t hmax = 0, pull = 0;
+    for (int i = 1; i <= k; ++i) {
+        const uint64_t rows = cr::binom(n + i - 1, i) * cr::binom(n + k - i - 1, k - i);
+        if (i < k) hmax = std::max(hmax, rows);
+        pull += rows;
+    }
+    it.hmax = static_cast<uint32_t>(hmax);
+    it.table_bytes = 2ull * it.C * n;
+    it.slot_words = 5ull * it.C + 2 * hmax + (it.C + 7) / 8;
+    it.cost = static_cast<double>(pull) * edges / n + static_cast<double>(it.C) * n;
+    return it.status = CR_OK;
+}
+
+// Device buffers and staging of one launch slot; they only grow.
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Buf d_graphs, d_order, d_masks, d_tables, d_out, d_scratch, d_next;
+    Buf h_graphs, h_order, h_masks, h_out;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint32_t count = 0, grid = 0;
+    std::vector<uint32_t> index;  // launched graph -> caller's index
+    std::vector<BatchGraph> graphs;
+    uint64_t total_table_bytes = 0, footprint = 0;
+    int max_grid = 0;
+
+    int init(int dev) {
+        device = dev;
+        HIP_TRY(hipSetDevice(dev));
+        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&ev0));
+        HIP_TRY(hipEventCreate(&ev1));
+        hipDeviceProp_t prop{};
+        HIP_TRY(hipGetDeviceProperties(&prop, dev));
+        max_grid = std::max(1, prop.multiProcessorCount) * 8;
+        return 0;
+    }
+    void release() {
+        for (Buf* b : {&d_graphs, &d_order, &d_masks, &d_tables, &d_out, &d_scratch, &d_next, &h_graphs, &h_order,
+                       &h_masks, &h_out})
+            drop(*b);
+        if (ev0) hipEventDestroy(ev0);
+        if (ev1) hipEventDestroy(ev1);
+        if (stream) hipStreamDestroy(stream);
+        ev0 = ev1 = nullptr;
+        stream = nullptr;
+    }
+
+    // Enqueues one launch over the valid items (asynchronous; finish() reads
+    // the results after the stream completes). sel holds caller indices.
+    int launch(const std::vector<const Item*>& items, const std::vector<uint32_t>& sel) {
+        HIP_TRY(hipSetDevice(device));
+        count = static_cast<uint32_t>(items.size());
+        index = sel;
+        graphs.assign(count, BatchGraph{});
+        uint64_t table_off = 0, mask_off = 0, slot_words = 64;  // bytes, masks, words
+        for (uint32_t i = 0; i < count; ++i) {
+            const Item& it = *items[i];
+            graphs[i] = BatchGraph{it.n, it.k, it.C, it.hmax, mask_off, table_off};
+            table_off += (it.table_bytes + 255) & ~255ull;
+            mask_off += it.n;
+            slot_words = std::max<uint64_t>(slot_words, (it.slot_words + 31) & ~31ull);
+        }
+        total_table_bytes = table_off;
+        std::vector<uint32_t> order(count);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return items[a]->cost > items[b]->cost; });
+        // workgroups: enough to fill the device, bounded by scratch memory
+        size_t freeb = 0, totalb = 0;
+        HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+        const uint64_t avail = freeb > table_off + (1ull << 30) ? (freeb - table_off - (1ull << 30)) / 2 : 0;
+        uint64_t g = std::min<uint64_t>(count, static_cast<uint64_t>(max_grid));
+        g = std::min<uint64_t>(g, std::max<uint64_t>(1, avail / (slot_words * 8)));
+        grid = static_cast<uint32_t>(std::max<uint64_t>(g, count ? 1 : 0));
+        int rc = 0;
+        if ((rc = grow(d_graphs, count * sizeof(BatchGraph), false)) || (rc = grow(d_order, count * 4ull, false)) ||
+            (rc = grow(d_masks, mask_off * 8, false)) || (rc = grow(d_tables, table_off, false)) ||
+            (rc = grow(d_out, count * sizeof(BatchOut), false)) ||
+            (rc = grow(d_scratch, static_cast<uint64_t>(grid) * slot_words * 8, false)) ||
+            (rc = grow(d_next, 64, false)) || (rc = grow(h_graphs, count * sizeof(BatchGraph), true)) ||
+            (rc = grow(h_order, count * 4ull, true)) || (rc = grow(h_masks, mask_off * 8, true)) ||
+            (rc = grow(h_out, count * sizeof(BatchOut), true)))
+            return rc;
+        footprint = table_off + static_cast<uint64_t>(grid) * slot_words * 8;
+        std::memcpy(h_graphs.p, graphs.data(), count * sizeof(BatchGraph));
+        std::memcpy(h_order.p, order.data(), count * 4ull);
+        uint64_t* hm = h_masks.as<uint64_t>();
+        for (uint32_t i = 0; i < count; ++i) std::memcpy(hm + graphs[i].mask_off, items[i]->masks, items[i]->n * 8ull);
+        if (count == 0) return 0;
+        HIP_TRY(hipMemcpyAsync(d_graphs.p, h_graphs.p, count * sizeof(BatchGraph), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(d_order.p, h_order.p, count * 4ull, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(d_masks.p, h_masks.p, mask_off * 8, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemsetAsync(d_next.p, 0, sizeof(unsigned int), stream));
+        HIP_TRY(hipEventRecord(ev0, stream));
+        hipLaunchKernelGGL(k_batch_solve, dim3(grid), dim3(kBlock), 0, stream, d_graphs.as<BatchGraph>(),
+                           d_order.as<uint32_t>(), count, d_masks.as<uint64_t>(), d_tables.as<uint8_t>(),
+                           d_out.as<BatchOut>(), d_scratch.as<uint64_t>(), slot_words, d_next.as<unsigned int>());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ev1, stream));
+        HIP_TRY(hipMemcpyAsync(h_out.p, d_out.p, count * sizeof(BatchOut), hipMemcpyDeviceToHost, stream));
+        return 0;
+    }
+
+    // Converts launched graph i's output (after the stream has completed).
+    void result(uint32_t i, const Item& it, cr_graph_result& r) const {
+        const BatchOut& o = h_out.as<BatchOut>()[i];
+        std::memset(&r, 0, sizeof r);
+        r.status = CR_OK;
+        r.base_captures = o.base;
+        r.won_states = o.won;
+        r.table_hash = o.hash;
+        r.table_bytes = it.table_bytes;
+        const uint32_t worst = static_cast<uint32_t>(o.verdict >> 32);
+        if (o.verdict != ~0ull && worst < CR_MAX_ROUND) {
+            r.win = 1;
+            r.plys = static_cast<int32_t>(worst);
+            r.best_cid = o.verdict & 0xffffffffull;
+            cr::unrank_tuple(r.best_cid, static_cast<int32_t>(it.n), static_cast<int32_t>(it.k), r.positions);
+        } else {
+            r.plys = -1;
+        }
+        r.n_passes = static_cast<int32_t>(pass_counts(o).size());
+    }
+
+    // The reference's per-pass counts from the depth histogram: pass i
+    // reports cop states of depth 2i-2, 2i-1 and robber states of depth 2i-1,
+    // 2i (depth > 0); the pass that finds nothing ends the loop.
+    static std::vector<uint64_t> pass_counts(const BatchOut& o) {
+        std::vector<uint64_t> pc;
+        for (int i = 1;; ++i) {
+            uint64_t c = 0;
+            const int c0 = 2 * i - 2, c1 = 2 * i - 1, r0 = 2 * i - 1, r1 = 2 * i;
+            if (c0 > 0 && c0 < CR_MAX_ROUND) c += o.hist[c0][0];
+            if (c1 < CR_MAX_ROUND) c += o.hist[c1][0];
+            if (r0 < CR_MAX_ROUND) c += o.hist[r0][1];
+            if (r1 < CR_MAX_ROUND) c += o.hist[r1][1];
+            pc.push_back(c);
+            if (c == 0) break;
+        }
+        return pc;
+    }
+};
+
+}  // namespace
+
+struct cr_batch {
+    Engine eng;
+    std::vector<BatchOut> last;  // outputs of the last solve, caller order
+    std::vector<uint8_t> valid;
+    double device_ms = 0;
+};
+
+struct cr_server {
+    struct Request {
+        uint64_t ticket;
+        uint64_t arrival_ns;
+        Item item;
+    };
+    struct Slot {
+        Engine eng;
+        bool busy = false;
+        std::vector<Request> reqs;
+    };
+    int device = 0;
+    uint64_t budget = 0, in_use = 0, peak = 0;
+    int32_t max_batch = 1024;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request> pending;
+    std::deque<cr_graph_result> done;
+    uint64_t next_ticket = 1, submitted = 0, finished = 0, launches = 0;
+    bool stopping = false;
+    std::thread th;
+    std::string thread_error;
+
+    void run();
+};
+
+void cr_server::run() {
+    hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        bool progress = false;
+        // harvest finished launches
+        for (auto& sp : slots) {
+            Slot& s = *sp;
+            if (!s.busy || hipEventQuery(s.eng.ev1) != hipSuccess) continue;
+            if (hipStreamQuery(s.eng.stream) != hipSuccess) continue;  // results still copying
+            const uint64_t t = now_ns();
+            for (uint32_t i = 0; i < s.reqs.size(); ++i) {
+                cr_graph_result r;
+                s.eng.result(i, s.reqs[i].item, r);
+                r.ticket = s.reqs[i].ticket;
+                r.latency_ms = static_cast<double>(t - s.reqs[i].arrival_ns) / 1e6;
+                done.push_back(r);
+            }
+            finished += s.reqs.size();
+            in_use -= s.eng.footprint;
+            s.reqs.clear();
+            s.busy = false;
+            progress = true;
+        }
+        if (progress) cv_done.notify_all();
+        // admit pending requests into a free slot
+        for (auto& sp : slots) {
+            Slot& s = *sp;
+            if (s.busy || pending.empty()) continue;
+            std::vector<Request> take;
+            uint64_t bytes = 0;
+            while (!pending.empty() && static_cast<int32_t>(take.size()) < max_batch) {
+                const Item& it = pending.front().item;
+                const uint64_t need = it.table_bytes + it.slot_words * 8;
+                if (in_use + bytes + need > budget && !take.empty()) break;
+                if (in_use + bytes + need > budget && in_use > 0) break;  // wait for memory to free up
+                bytes += need;
+                take.push_back(std::move(pending.front()));
+                pending.pop_front();
+            }
+            if (take.empty()) break;
+            s.reqs = std::move(take);
+            std::vector<const Item*> items;
+            std::vector<uint32_t> sel;
+            for (uint32_t i = 0; i < s.reqs.size(); ++i) {
+                items.push_back(&s.reqs[i].item);
+                sel.push_back(i);
+            }
+            lk.unlock();
+            const int rc = s.eng.launch(items, sel);
+            lk.lock();
+            if (rc != 0) {  // report every request of the failed launch
+                for (auto& rq : s.reqs) {
+                    cr_graph_result r{};
+                    r.ticket = rq.ticket;
+                    r.status = rc;
+                    r.plys = -1;
+                    done.push_back(r);
+                }
+                finished += s.reqs.size();
+                s.reqs.clear();
+                cv_done.notify_all();
+                continue;
+            }
+            s.busy = true;
+            in_use += s.eng.footprint;
+            peak = std::max(peak, in_use);
+            ++launches;
+            progress = true;
+        }
+        bool busy = false;
+        for (auto& sp : slots) busy = busy || sp->busy;
+        if (stopping && !busy && pending.empty()) break;
+        if (!progress) {
+            if (busy) cv_work.wait_for(lk, std::chrono::microseconds(20));
+            else cv_work.wait_for(lk, std::chrono::milliseconds(5));
+        }
+    }
+}
+
+extern "C" {
+
+int cr_batch_create(int device, cr_batch** out) {
+    if (out == nullptr) return fail(CR_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CR_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count) return fail(CR_ERR_ARG, "device %d of %d", device, count);
+    auto b = std::make_unique<cr_batch>();
+    if (int rc = b->eng.init(device)) return rc;
+    *out = b.release();
+    return CR_OK;
+}
+
+void cr_batch_destroy(cr_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->eng.device);
+    if (b->eng.stream) hipStreamSynchronize(b->eng.stream);
+    b->eng.release();
+    delete b;
+}
+
+int cr_batch_solve(cr_batch* b, int32_t count, const int32_t* n, const int32_t* k, const uint8_t* adj,
+                   const uint64_t* adj_offset, cr_graph_result* results, uint8_t* tables, const uint64_t* table_offset) {
+    if (b == nullptr) return fail(CR_ERR_ARG, "null batch");
+    if (count < 0 || (count > 0 && (n == nullptr || k == nullptr || adj == nullptr || adj_offset == nullptr ||
+                                    results == nullptr)))
+        return fail(CR_ERR_ARG, "null argument");
+    if (tables != nullptr && table_offset == nullptr) return fail(CR_ERR_ARG, "tables without offsets");
+    std::vector<Item> items(count);
+    std::vector<const Item*> ok;
+    std::vector<uint32_t> sel;
+    int first_bad = CR_OK;
+    char first_msg[512] = "";
+    for (int32_t i = 0; i < count; ++i) {
+        const int st = prepare(items[i], n[i], k[i], adj + adj_offset[i]);
+        std::memset(&results[i], 0, sizeof results[i]);
+        results[i].ticket = static_cast<uint64_t>(i);
+        results[i].status = st;
+        results[i].plys = -1;
+        if (st == CR_OK) {
+            ok.push_back(&items[i]);
+            sel.push_back(static_cast<uint32_t>(i));
+        } else if (first_bad == CR_OK) {
+            first_bad = st;
+            std::snprintf(first_msg, sizeof first_msg, "graph %d: %s", i, crd::g_err);
+        }
+    }
+    b->last.assign(count, BatchOut{});
+    b->valid.assign(count, 0);
+    Engine& e = b->eng;
+    if (int rc = e.launch(ok, sel)) return rc;
+    HIP_TRY(hipStreamSynchronize(e.stream));
+    float ms = 0;
+    if (!ok.empty()) hipEventElapsedTime(&ms, e.ev0, e.ev1);
+    b->device_ms = ms;
+    for (uint32_t i = 0; i < ok.size(); ++i) {
+        const uint32_t ci = sel[i];
+        e.result(i, *ok[i], results[ci]);
+        results[ci].ticket = ci;
+        b->last[ci] = e.h_out.as<BatchOut>()[i];
+        b->valid[ci] = 1;
+        if (tables)
+            HIP_TRY(hipMemcpy(tables + table_offset[ci], e.d_tables.as<uint8_t>() + e.graphs[i].table_off,
+                              ok[i]->table_bytes, hipMemcpyDeviceToHost));
+    }
+    if (first_bad != CR_OK) return fail(CR_ERR_ARG, "%s", first_msg);
+    return CR_OK;
+}
+
+int cr_batch_pass_counts(cr_batch* b, int32_t index, uint64_t* out, int32_t cap, int32_t* n_passes) {
+    if (b == nullptr) return fail(CR_ERR_ARG, "null batch");
+    if (index < 0 || index >= static_cast<int32_t>(b->last.size()) || !b->valid[index])
+        return fail(CR_ERR_ARG, "no solved graph %d in the last batch", index);
+    const std::vector<uint64_t> pc = Engine::pass_counts(b->last[index]);
+    for (int32_t i = 0; out && i < cap && i < static_cast<int32_t>(pc.size()); ++i) out[i] = pc[i];
+    if (n_passes) *n_passes = static_cast<int32_t>(pc.size());
+    return CR_OK;
+}
+
+int cr_batch_last_timing(cr_batch* b, double* device_ms, int32_t* workgroups) {
+    if (b == nullptr) return fail(CR_ERR_ARG, "null batch");
+    if (device_ms) *device_ms = b->device_ms;
+    if (workgroups) *workgroups = static_cast<int32_t>(b->eng.grid);
+    return CR_OK;
+}
+
+uint64_t cr_clock_ns(void) { return now_ns(); }
+
+int cr_server_create(int device, uint64_t hbm_budget, int32_t max_batch, int32_t streams, cr_server** out) {
+    if (out == nullptr) return fail(CR_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (streams < 1 || streams > 4) return fail(CR_ERR_ARG, "streams must be 1..4 (got %d)", streams);
+    if (max_batch < 1) return fail(CR_ERR_ARG, "max_batch must be positive");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CR_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count) return fail(CR_ERR_ARG, "device %d of %d", device, count);
+    auto s = std::make_unique<cr_server>();
+    s->device = device;
+    s->budget = hbm_budget;
+    s->max_batch = max_batch;
+    for (int i = 0; i < streams; ++i) {
+        auto sl = std::make_unique<cr_server::Slot>();
+        if (int rc = sl->eng.init(device)) return rc;
+        s->slots.push_back(std::move(sl));
+    }
+    cr_server* raw = s.get();
+    s->th = std::thread([raw] { raw->run(); });
+    *out = s.release();
+    return CR_OK;
+}
+
+int cr_server_submit(cr_server* s, int32_t n, int32_t k, const uint8_t* adj, uint64_t arrival_ns, uint64_t* ticket) {
+    if (s == nullptr) return fail(CR_ERR_ARG, "null server");
+    cr_server::Request rq;
+    if (int rc = prepare(rq.item, n, k, adj)) return rc;
+    const uint64_t need = rq.item.table_bytes + rq.item.slot_words * 8;
+    if (need > s->budget) return fail(CR_ERR_TOO_LARGE, "request needs %llu bytes; the budget is %llu",
+                                      static_cast<unsigned long long>(need), static_cast<unsigned long long>(s->budget));
+    rq.arrival_ns = arrival_ns ? arrival_ns : now_ns();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stopping) return fail(CR_ERR_ORDER, "server is stopping");
+        rq.ticket = s->next_ticket++;
+        if (ticket) *ticket = rq.ticket;
+        s->pending.push_back(std::move(rq));
+        ++s->submitted;
+    }
+    s->cv_work.notify_one();
+    return CR_OK;
+}
+
+int cr_server_submit_many(cr_server* s, int32_t count, const int32_t* n, const int32_t* k, const uint8_t* adj,
+                          const uint64_t* adj_offset, const uint64_t* arrival_ns, uint64_t* tickets,
+                          int32_t* status) {
+    if (s == nullptr) return fail(CR_ERR_ARG, "null server");
+    if (count < 0 || (count > 0 && (n == nullptr || k == nullptr || adj == nullptr || adj_offset == nullptr ||
+                                    tickets == nullptr)))
+        return fail(CR_ERR_ARG, "null argument");
+    std::vector<cr_server::Request> reqs;
+    reqs.reserve(count);
+    std::vector<int32_t> st(count, CR_OK);
+    const uint64_t now = now_ns();
+    for (int32_t i = 0; i < count; ++i) {
+        cr_server::Request rq;
+        st[i] = prepare(rq.item, n[i], k[i], adj + adj_offset[i]);
+        if (st[i] == CR_OK && rq.item.table_bytes + rq.item.slot_words * 8 > s->budget)
+            st[i] = CR_ERR_TOO_LARGE;
+        rq.arrival_ns = arrival_ns && arrival_ns[i] ? arrival_ns[i] : now;
+        rq.ticket = 0;
+        reqs.push_back(std::move(rq));
+    }
+    bool rejected = false;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stopping) return fail(CR_ERR_ORDER, "server is stopping");
+        for (int32_t i = 0; i < count; ++i) {
+            if (status) status[i] = st[i];
+            if (st[i] != CR_OK) {
+                tickets[i] = 0;
+                rejected = true;
+                continue;
+            }
+            reqs[i].ticket = s->next_ticket++;
+            tickets[i] = reqs[i].ticket;
+            s->pending.push_back(std::move(reqs[i]));
+            ++s->submitted;
+        }
+    }
+    s->cv_work.notify_one();
+    return rejected ? fail(CR_ERR_ARG, "some requests were rejected (see status)") : CR_OK;
+}
+
+int cr_server_poll(cr_server* s, cr_graph_result* out, int32_t cap, int32_t wait_us, int32_t* count) {
+    if (s == nullptr || (cap > 0 && out == nullptr)) return fail(CR_ERR_ARG, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && wait_us > 0)
+        s->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [s] { return !s->done.empty(); });
+    int32_t m = 0;
+    while (m < cap && !s->done.empty()) {
+        out[m++] = s->done.front();
+        s->done.pop_front();
+    }
+    if (count) *count = m;
+    return CR_OK;
+}
+
+int cr_server_stats(cr_server* s, uint64_t* submitted, uint64_t* finished, uint64_t* launches, uint64_t* peak_bytes) {
+    if (s == nullptr) return fail(CR_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (submitted) *submitted = s->submitted;
+    if (finished) *finished = s->finished;
+    if (launches) *launches = s->launches;
+    if (peak_bytes) *peak_bytes = s->peak;
+    return CR_OK;
+}
+
+void cr_server_destroy(cr_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    if (s->th.joinable()) s->th.join();
+    hipSetDevice(s->device);
+    for (auto& sl : s->slots) {
+        if (sl->eng.stream) hipStreamSynchronize(sl->eng.stream);
+        sl->eng.release();
+    }
+    delete s;
+}
+
+}  // extern "C"

@@ -141,6 +141,31 @@ __device__ __forceinline__ bool product_tuple(uint32_t t, const uint32_t* cops, 
 }
 
 
+// rank of a sorted j-tuple over [0, n): C(M,j) - 1 - sum_i C(M-1-(t_i+i), j-i), M = n+j-1
+__device__ __forceinline__ uint32_t rank_j(const uint32_t* t, int j, int n, const uint32_t* binom, int Mimg) {
+    const int M = n + j - 1;
+    uint32_t s = 0;
+    for (int i = 0; i < j; ++i) s += binom[(j - i) * (Mimg + 1) + (M - 1 - static_cast<int>(t[i]) - i)];
+    return binom[j * (Mimg + 1) + M] - 1 - s;
+}
+
+__device__ __forceinline__ void unrank_j(uint32_t rank, int j, int n, const uint32_t* binom, int Mimg, uint32_t* out) {
+    const int M = n + j - 1;
+    uint32_t s = binom[j * (Mimg + 1) + M] - 1 - rank;
+    int xmax = M - 1;
+    for (int i = 0; i < j; ++i) {
+        const uint32_t* row = binom + (j - i) * (Mimg + 1);
+        int lo = 0, hi = xmax;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (row[mid] <= s) lo = mid; else hi = mid - 1;
+        }
+        s -= row[lo];
+        out[i] = static_cast<uint32_t>(M - 1 - lo - i);
+        xmax = lo - 1;
+    }
+}
+
 // Block-wide sums of wave-uniform per-wave values; thread 0 gets the totals.
 // Every thread of the block must call it.
 __device__ __forceinline__ void block_sum2(unsigned long long& x, unsigned long long& y) {
@@ -189,7 +214,7 @@ __device__ unsigned long long block_exclusive_scan(unsigned long long v, unsigne
     return r;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_tile_sums(const uint32_t* cnt, uint64_t n,
+[[maybe_unused]] __global__ __launch_bounds__(kScanThreads) void k_tile_sums(const uint32_t* cnt, uint64_t n,
                                                             unsigned long long* tiles) {
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * 8ull;
     unsigned long long v = 0;
@@ -201,7 +226,7 @@ __global__ __launch_bounds__(kScanThreads) void k_tile_sums(const uint32_t* cnt,
 }
 
 // Scans the tile sums in place and stores the grand total as heads[n].
-__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(unsigned long long* tiles, uint32_t ntiles,
+[[maybe_unused]] __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(unsigned long long* tiles, uint32_t ntiles,
                                                              unsigned long long* heads, uint64_t n) {
     unsigned long long carry = 0;
     for (uint32_t b0 = 0; b0 < ntiles; b0 += kScanThreads) {
@@ -215,7 +240,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(unsigned long long*
     if (threadIdx.x == 0) heads[n] = carry;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_tile_apply(const uint32_t* cnt, uint64_t n,
+[[maybe_unused]] __global__ __launch_bounds__(kScanThreads) void k_tile_apply(const uint32_t* cnt, uint64_t n,
                                                              const unsigned long long* tiles,
                                                              unsigned long long* heads) {
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * 8ull;
@@ -248,7 +273,7 @@ T* dev_alloc(uint64_t count, uint64_t& total, int& rc) {
 
 // Kernels whose dynamic LDS (the graph image) exceeds the 64 KiB default need
 // the per-function limit raised before launch.
-int raise_lds_limit(const void* fn, uint32_t bytes) {
+inline int raise_lds_limit(const void* fn, uint32_t bytes) {
     if (bytes <= 64 * 1024) return 0;
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
     return 0;

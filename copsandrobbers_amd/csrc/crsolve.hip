@@ -47,6 +47,7 @@
 #include <vector>
 
 #include "cr_device.h"
+#include "cr_columns.h"
 #include "cr_host.h"
 #include "cr_macro.h"
 #include "crsolve.h"
@@ -343,31 +344,6 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
 // read O(sum |H_i| * degree) rows instead of O(C * |T|) random ones; for a
 // fixed A the reads for increasing B are increasing rows of H_{i-1}.
 // ---------------------------------------------------------------------------
-// rank of a sorted j-tuple over [0, n): C(M,j) - 1 - sum_i C(M-1-(t_i+i), j-i), M = n+j-1
-__device__ __forceinline__ uint32_t rank_j(const uint32_t* t, int j, int n, const uint32_t* binom, int Mimg) {
-    const int M = n + j - 1;
-    uint32_t s = 0;
-    for (int i = 0; i < j; ++i) s += binom[(j - i) * (Mimg + 1) + (M - 1 - static_cast<int>(t[i]) - i)];
-    return binom[j * (Mimg + 1) + M] - 1 - s;
-}
-
-__device__ __forceinline__ void unrank_j(uint32_t rank, int j, int n, const uint32_t* binom, int Mimg, uint32_t* out) {
-    const int M = n + j - 1;
-    uint32_t s = binom[j * (Mimg + 1) + M] - 1 - rank;
-    int xmax = M - 1;
-    for (int i = 0; i < j; ++i) {
-        const uint32_t* row = binom + (j - i) * (Mimg + 1);
-        int lo = 0, hi = xmax;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (row[mid] <= s) lo = mid; else hi = mid - 1;
-        }
-        s -= row[lo];
-        out[i] = static_cast<uint32_t>(M - 1 - lo - i);
-        xmax = lo - 1;
-    }
-}
-
 struct SepStage {
     const uint64_t* in;   // H_{i-1}, rows [cnt(i-1) * cnt(K-i+1)]
     uint64_t* out;        // H_i, rows [cnt(i) * cnt(K-i)] (stage K: nx0)
@@ -962,6 +938,9 @@ struct cr_solver {
     crd::MacroPlan* macro = nullptr;
     uint64_t* d_occ = nullptr;  // cop mask per configuration
     crd::MacroProfile mprof;
+    // column solver for p >= 2 (columns.hip)
+    crd::ColumnsPlan* colplan = nullptr;
+    std::vector<uint64_t> reach;  // [n][4]: vertices the robber reaches from r in exactly p moves
 
     uint32_t nbatch() const { return static_cast<uint32_t>((C + 63) / 64); }
     uint64_t ngroups() const { return (nbatch() + kGroupWords - 1) / kGroupWords; }
@@ -1011,6 +990,8 @@ struct cr_solver {
         csr_entries = 0;
         crd::macro_free(macro, device_bytes);
         macro = nullptr;
+        crd::columns_free(colplan, device_bytes);
+        colplan = nullptr;
         hipFree(d_occ);
         d_occ = nullptr;
         hipFree(d_ctl);
@@ -1397,6 +1378,40 @@ int build_macro(cr_solver* s) {
     return 0;
 }
 
+// Column solver with p >= 2: the 2p-column table and the p-step reach masks;
+// the planes are allocated by initializeCaptures.
+int build_columns(cr_solver* s) {
+    const int n = s->g.n;
+    if (2 * s->p > 126) return fail(CR_ERR_UNSUPPORTED, "p=%d: the device column solver keeps at most 126 columns", s->p);
+    s->columns = 2 * s->p;
+    s->S = s->C * static_cast<uint64_t>(n) * s->columns;
+    s->W = (n + 63) / 64;
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    if (s->S + (4 << 20) > freeb) return fail(CR_ERR_TOO_LARGE, "state space needs %llu bytes; %zu free", static_cast<unsigned long long>(s->S), freeb);
+    int rc = 0;
+    if (!(s->d_table = dev_alloc<uint8_t>(s->S, s->device_bytes, rc))) return rc;
+    if (!(s->d_ctl = dev_alloc<Ctl>(1, s->device_bytes, rc))) return rc;
+    if (!s->h_ctl) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_ctl), sizeof(Ctl), hipHostMallocDefault));
+    if ((rc = upload_graph(s)) != 0) return rc;
+    // exactly-p-step reach sets (k_cops_visibility.cpp:235-288): robber moves along out-edges
+    s->reach.assign(static_cast<size_t>(n) * 4, 0);
+    for (int r = 0; r < n; ++r) {
+        uint64_t cur[4] = {0, 0, 0, 0};
+        cur[r >> 6] = 1ull << (r & 63);
+        for (int step = 0; step < s->p; ++step) {
+            uint64_t nxt[4] = {0, 0, 0, 0};
+            for (int v = 0; v < n; ++v)
+                if ((cur[v >> 6] >> (v & 63)) & 1)
+                    for (uint8_t u : s->g.out[v]) nxt[u >> 6] |= 1ull << (u & 63);
+            std::memcpy(cur, nxt, sizeof cur);
+        }
+        std::memcpy(&s->reach[static_cast<size_t>(r) * 4], cur, sizeof cur);
+    }
+    s->stage = kBuilt;
+    return 0;
+}
+
 int do_build(cr_solver* s) {
     if (s->stage < kLoaded) return fail(CR_ERR_ORDER, "buildAuxGraph before loadGraphFile");
     s->release();
@@ -1404,8 +1419,6 @@ int do_build(cr_solver* s) {
     s->C = cr::config_count(n, s->k);
     if (s->C == 0) return fail(CR_ERR_K_RANGE, "no configuration for k=%d", s->k);
     if (s->C >= 0xffffffffull) return fail(CR_ERR_TOO_LARGE, "%llu configurations exceed 32-bit ids", static_cast<unsigned long long>(s->C));
-    if (s->variant == CR_VARIANT_COLUMNS && s->p != 1)
-        return fail(CR_ERR_UNSUPPORTED, "the column solver runs on the device for p = 1 only");
     uint64_t prod = 1, prod_out = 1;
     for (int i = 0; i < s->k; ++i) {
         prod *= static_cast<uint64_t>(std::max(1, s->g.max_in));
@@ -1414,6 +1427,7 @@ int do_build(cr_solver* s) {
     if (prod >= (1ull << 32) || prod_out >= (1ull << 32))
         return fail(CR_ERR_TOO_LARGE, "team-move product %llu too large", static_cast<unsigned long long>(std::max(prod, prod_out)));
     if (s->variant == CR_VARIANT_MACRO) return build_macro(s);
+    if (s->p > 1) return build_columns(s);
     s->columns = 2 * s->p;
     s->S = s->C * static_cast<uint64_t>(n) * s->columns;
     s->W = (n + 63) / 64;
@@ -1475,6 +1489,51 @@ crd::TimeSink time_sink(cr_solver* s) {
     return t;
 }
 
+int init_columns(cr_solver* s, uint64_t* base) {
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    uint64_t caps = 0;
+    const int rc = crd::columns_init(&s->colplan, s->gv, s->reach, s->k, s->p, s->stream, s->d_table, s->device_bytes, &caps);
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    s->stats = cr_stats{};
+    s->stats.init_ms = ms;
+    s->stats.base_captures = caps;
+    if (base) *base = caps;
+    s->stage = kInitialized;
+    return 0;
+}
+
+int loop_columns(cr_solver* s) {
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    uint64_t won = 0;
+    int32_t levels = 0;
+    const int rc = crd::columns_solve(s->colplan, time_sink(s), s->pass_counts, &won, &levels);
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    s->stats.loop_ms = ms;
+    s->stats.won_states = won;
+    s->stats.n_levels = levels;
+    s->stats.n_passes = static_cast<int32_t>(s->pass_counts.size());
+    s->stage = kSolved;
+    return 0;
+}
+
 int init_macro(cr_solver* s, uint64_t* base) {
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -1502,6 +1561,7 @@ int do_init(cr_solver* s, uint64_t* base) {
     int rc = reset_ctl(s);
     if (rc) return rc;
     if (s->variant == CR_VARIANT_MACRO) return init_macro(s, base);
+    if (s->p > 1) return init_columns(s, base);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1565,6 +1625,7 @@ int do_loop(cr_solver* s) {
     s->ktimes.clear();
     s->korder.clear();
     if (s->variant == CR_VARIANT_MACRO) return loop_macro(s);
+    if (s->p > 1) return loop_columns(s);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1808,6 +1869,7 @@ int cr_find_final_result(cr_solver* s, cr_verdict* out) {
     const unsigned long long init = ~0ull;
     HIP_TRY(hipMemcpyAsync(&s->d_ctl->verdict, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
     LevelArgs a = level_args(s);
+    if (s->colplan) a.won0 = crd::columns_won0(s->colplan);  // column 0 of the p >= 2 solver
     if (int rc = dispatch_w<LaunchVerdict>(s->W, s, a)) return rc;
     HIP_TRY(hipGetLastError());
     unsigned long long v = 0;
@@ -1885,7 +1947,8 @@ int cr_copy_configs(cr_solver* s, uint8_t* dst, uint64_t first, uint64_t count) 
 
 int cr_level_counts(cr_solver* s, uint64_t* out) {
     if (int rc = check_solver(s)) return rc;
-    if (s->variant == CR_VARIANT_MACRO) return fail(CR_ERR_UNSUPPORTED, "depth levels belong to the column solver");
+    if (s->variant == CR_VARIANT_MACRO || s->p != 1)
+        return fail(CR_ERR_UNSUPPORTED, "depth levels and retro-edges are kept for the p = 1 column solver");
     if (out == nullptr) return fail(CR_ERR_ARG, "null output");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
     for (int d = 0; d < 127; ++d) {
@@ -1897,7 +1960,8 @@ int cr_level_counts(cr_solver* s, uint64_t* out) {
 
 int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_t* won) {
     if (int rc = check_solver(s)) return rc;
-    if (s->variant == CR_VARIANT_MACRO) return fail(CR_ERR_UNSUPPORTED, "depth levels belong to the column solver");
+    if (s->variant == CR_VARIANT_MACRO || s->p != 1)
+        return fail(CR_ERR_UNSUPPORTED, "depth levels and retro-edges are kept for the p = 1 column solver");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
     unsigned long long r[3];
     if (int rc = count_moves(s, 0, r)) return rc;
@@ -1909,7 +1973,8 @@ int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_
 
 int cr_work_profile(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint8_t* pulled) {
     if (int rc = check_solver(s)) return rc;
-    if (s->variant == CR_VARIANT_MACRO) return fail(CR_ERR_UNSUPPORTED, "depth levels belong to the column solver");
+    if (s->variant == CR_VARIANT_MACRO || s->p != 1)
+        return fail(CR_ERR_UNSUPPORTED, "depth levels and retro-edges are kept for the p = 1 column solver");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
     unsigned long long r[3];
     if (int rc = count_moves(s, 0, r)) return rc;

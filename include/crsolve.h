@@ -130,7 +130,6 @@ int cr_find_final_result(cr_solver* s, cr_verdict* out);
  * states to path. */
 int cr_save_aux_graph(cr_solver* s, const char* path);
 
-
 /* ---- accessors ------------------------------------------------------- */
 int cr_get_info(const cr_solver* s, cr_info* out);
 int cr_get_stats(const cr_solver* s, cr_stats* out);
@@ -192,6 +191,77 @@ int cr_kernel_times(cr_solver* s, char* names_out, int32_t names_cap, double* ms
  * '\0') or a negative status. */
 int cr_cache_key(const char* graph_path, int32_t k, int32_t p, int32_t variant, char* out,
                  int32_t cap);
+
+/* ---- many small graphs: one launch, and a continuous-batching server ---
+ * The plain game (full information, p = 1) of many independent graphs. The
+ * reference solves one graph per process (app/main.cpp:92-111, swept by
+ * util/test_k_cops_visibility.py:121-145); these entry points have no
+ * counterpart there and serve callers that solve graphs in bulk or as a
+ * stream of requests. Every result equals the six-step solve of that graph
+ * (tables, passes, verdict). */
+#define CR_BATCH_MAX_N 64 /* vertices per graph in a batch */
+#define CR_BATCH_MAX_K 4  /* cops per graph in a batch */
+
+typedef struct {
+    uint64_t ticket;        /* cr_server_submit's ticket (index for cr_batch_solve) */
+    int32_t status;         /* CR_OK, or the error that rejected this graph */
+    int32_t win;            /* findFinalResult: 1 = WIN */
+    int32_t plys;           /* capture time when win, else -1 */
+    int32_t n_passes;       /* passes the reference prints (the last reports 0) */
+    uint64_t best_cid;      /* optimal start configuration when win */
+    uint8_t positions[CR_MAX_K];
+    uint64_t base_captures; /* "Initialized n base captures." */
+    uint64_t won_states;    /* marked states of both columns */
+    uint64_t table_hash;    /* sum over state bytes b_i of (b_i + 1) * mix64(i): equal tables, equal hash */
+    uint64_t table_bytes;   /* 2 * C * n */
+    double latency_ms;      /* cr_server: submit (or stated arrival) to completion; 0 otherwise */
+} cr_graph_result;
+
+typedef struct cr_batch cr_batch;
+int cr_batch_create(int device, cr_batch** out);
+void cr_batch_destroy(cr_batch* b);
+/* Solves count graphs in one persistent launch. Graph i has n[i] <= 64
+ * vertices, k[i] <= 4 cops and its n[i]*n[i] 0/1 matrix (Graph.cpp layout,
+ * row = moves out of a vertex) at adj + adj_offset[i]. results[i] receives its
+ * result. When tables is not null, graph i's state table (the reference's
+ * layout, results[i].table_bytes bytes) is copied to tables + table_offset[i].
+ * Graphs that fail validation get their status in results[i].status and the
+ * call returns CR_ERR_ARG after solving the others. */
+int cr_batch_solve(cr_batch* b, int32_t count, const int32_t* n, const int32_t* k, const uint8_t* adj,
+                   const uint64_t* adj_offset, cr_graph_result* results, uint8_t* tables,
+                   const uint64_t* table_offset);
+/* Per-pass "Found n" counts of graph `index` of the last cr_batch_solve. */
+int cr_batch_pass_counts(cr_batch* b, int32_t index, uint64_t* out, int32_t cap, int32_t* n_passes);
+/* Device time of the last cr_batch_solve (launch to last result) and its grid. */
+int cr_batch_last_timing(cr_batch* b, double* device_ms, int32_t* workgroups);
+
+/* Continuous batching: a dispatcher thread admits pending requests into a new
+ * launch whenever one of `streams` launch slots is free and their footprint
+ * (state table + per-graph work planes, bytes) fits in what `hbm_budget`
+ * leaves after the launches in flight; launches on different slots overlap
+ * on the device. Results come back in completion order. */
+typedef struct cr_server cr_server;
+int cr_server_create(int device, uint64_t hbm_budget, int32_t max_batch, int32_t streams, cr_server** out);
+/* Copies the request; arrival_ns (cr_clock_ns time base, 0 = now) is when it
+ * arrived, from which its latency is measured. */
+int cr_server_submit(cr_server* s, int32_t n, int32_t k, const uint8_t* adj, uint64_t arrival_ns,
+                     uint64_t* ticket);
+/* Submits count requests in one call (graph i laid out as in cr_batch_solve;
+ * arrival_ns may be null = now). tickets[i] gets request i's ticket, or 0
+ * when it was rejected, with the reason in status[i] (may be null). Returns
+ * CR_OK, or CR_ERR_ARG when some request was rejected. */
+int cr_server_submit_many(cr_server* s, int32_t count, const int32_t* n, const int32_t* k, const uint8_t* adj,
+                          const uint64_t* adj_offset, const uint64_t* arrival_ns, uint64_t* tickets,
+                          int32_t* status);
+/* Moves up to cap finished results to out; waits up to wait_us for the first. */
+int cr_server_poll(cr_server* s, cr_graph_result* out, int32_t cap, int32_t wait_us, int32_t* count);
+/* Requests submitted, finished, launches made, peak bytes admitted at once. */
+int cr_server_stats(cr_server* s, uint64_t* submitted, uint64_t* finished, uint64_t* launches,
+                    uint64_t* peak_bytes);
+/* Finishes the work in flight and stops the dispatcher. */
+void cr_server_destroy(cr_server* s);
+/* Steady clock in nanoseconds (the arrival_ns time base). */
+uint64_t cr_clock_ns(void);
 
 #ifdef __cplusplus
 }

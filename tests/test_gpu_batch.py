@@ -1,0 +1,236 @@
+"""Batched solving and the continuous-batching server against the reference's
+tables and the six-step single-graph solve.
+
+Expected values: the compiled reference's p = 1 cache blobs (tests/golden/ref)
+for every golden graph that fits a batch (n <= 64, k <= 4), and the
+single-graph device solve (itself checked against the reference and the
+oracle in test_gpu_parity.py) for seeded random graphs. Integer work: every
+comparison is exact.
+"""
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, matrix
+
+import copsandrobbers_amd as cra
+from copsandrobbers_amd import batch as crb
+from copsandrobbers_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+with open(os.path.join(GOLDEN, "ref_index.json")) as f:
+    INDEX = json.load(f)
+
+
+def say(*args):
+    print(*args, flush=True)
+    sys.stdout.flush()
+
+
+def mix64(x):
+    x = np.asarray(x, np.uint64)
+    with np.errstate(over="ignore"):
+        x = x ^ (x >> np.uint64(30))
+        x = x * np.uint64(0xbf58476d1ce4e5b9)
+        x = x ^ (x >> np.uint64(27))
+        x = x * np.uint64(0x94d049bb133111eb)
+        x = x ^ (x >> np.uint64(31))
+    return x
+
+
+def table_hash(states):
+    """The documented table hash: sum over i of (byte_i + 1) * mix64(i) mod 2^64."""
+    i = np.arange(states.size, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        return int(np.sum((states.astype(np.uint64) + np.uint64(1)) * mix64(i), dtype=np.uint64))
+
+
+def single(solver, adj, k):
+    solver.load_graph_matrix(adj, k, 1)
+    solver.build_aux_graph()
+    base = solver.initialize_captures()
+    passes = solver.main_loop()
+    verdict = solver.find_final_result()
+    return base, passes, verdict, solver.states(), solver.stats().won_states
+
+
+@pytest.fixture(scope="module")
+def batch():
+    b = crb.Batch(0)
+    yield b
+    b.close()
+
+
+@pytest.fixture(scope="module")
+def solver():
+    s = cra.Solver(0)
+    yield s
+    s.close()
+
+
+def golden_cases():
+    out = []
+    for name, e in sorted(INDEX.items()):
+        if e["solver"] != "v1" or e["p"] != 1 or e["k"] > crb.MAX_BATCH_K:
+            continue
+        n, adj = cra.parse_matrix(open(matrix(e["graph"]), "rb").read())
+        if n <= crb.MAX_N and adj.sum(1).min() > 0:
+            out.append((name, adj, e["k"]))
+    return out
+
+
+def test_batch_matches_reference_tables(batch):
+    cases = golden_cases()
+    say("batch golden", len(cases))
+    res, tabs = batch.solve([(adj, k) for _, adj, k in cases], tables=True)
+    for i, (name, adj, k) in enumerate(cases):
+        e = INDEX[name]
+        r = res[i]
+        assert r["status"] == 0, name
+        if e["arrays"]:
+            g = np.load(os.path.join(GOLDEN, "ref", e["arrays"]))
+            assert np.array_equal(tabs[i], g["states"]), name
+        assert hashlib.sha256(tabs[i].tobytes()).hexdigest() == e["states_sha256"], name
+        assert r["table_hash"] == table_hash(tabs[i]), name
+        assert batch.pass_counts(i) == e["passes"], name
+        assert r["n_passes"] == len(e["passes"]), name
+        assert r["base_captures"] == e["base_caps"], name
+        if e["result"] == "WIN":
+            assert r["win"] and r["plys"] == e["plys"], name
+            assert ", ".join(str(x) for x in r["positions"]) == e["positions"], name
+        else:
+            assert not r["win"], name
+
+
+def random_graphs(count, seed_base):
+    graphs = workloads.batch_graphs(count, seed_base=seed_base)
+    # a directed graph (every vertex keeps an out-edge) and one with stay moves
+    rng = workloads.SplitMix64(77)
+    n = 13
+    di = np.zeros((n, n), np.uint8)
+    for u in range(n):
+        di[u, (u + 3) % n] = 1
+        v = rng.below(n)
+        if v != u:
+            di[u, v] = 1
+    graphs.append((di, 2))
+    loops, _ = workloads.er_graph(11, 0.25, 5)
+    loops = loops.copy()
+    loops[np.arange(11), np.arange(11)] = 1
+    graphs.append((loops, 3))
+    path64 = np.zeros((64, 64), np.uint8)
+    for u in range(63):
+        path64[u, u + 1] = path64[u + 1, u] = 1
+    graphs.append((path64, 1))
+    k4, _ = workloads.er_graph(12, 0.3, 2)
+    graphs.append((k4, 4))
+    return graphs
+
+
+def test_batch_matches_single_graph_solves(batch, solver, oracle):
+    graphs = random_graphs(48, seed_base=5000)
+    say("batch random", len(graphs))
+    res, tabs = batch.solve(graphs, tables=True)
+    for i, (adj, k) in enumerate(graphs[-4:], start=len(graphs) - 4):  # directed, stay moves, n = 64, k = 4
+        want = oracle.v1_solve(adj, k, 1)
+        assert np.array_equal(tabs[i], want.states), i
+        assert batch.pass_counts(i) == want.pass_counts, i
+    for i, (adj, k) in enumerate(graphs):
+        base, passes, verdict, states, won = single(solver, adj, k)
+        r = res[i]
+        assert np.array_equal(tabs[i], states), i
+        assert batch.pass_counts(i) == passes, i
+        assert r["base_captures"] == base and r["won_states"] == won, i
+        assert r["win"] == (verdict is not None), i
+        if verdict is not None:
+            assert r["plys"] == verdict["plys"] and r["positions"] == verdict["positions"], i
+
+
+def test_batch_reports_invalid_graphs_and_solves_the_rest(batch):
+    good, _ = workloads.er_graph(9, 0.4, 1)
+    iso = good.copy()
+    iso[4, :] = 0
+    big = np.ones((65, 65), np.uint8)
+    graphs = [(good, 2), (iso, 2), (good, 5), (big, 1), (good, 1)]
+    say("batch errors")
+    res = batch.solve(graphs, raise_on_error=False)
+    assert [r["status"] for r in res] == [0, -5, -6, -4, 0]
+    alone = batch.solve([(good, 2), (good, 1)])
+    assert res[0]["table_hash"] == alone[0]["table_hash"] and res[4]["table_hash"] == alone[1]["table_hash"]
+    with pytest.raises(cra.SolverError) as exc:
+        batch.solve(graphs)
+    assert exc.value.code == -1
+    assert batch.solve([]) == []
+
+
+def test_server_results_equal_alone_and_in_the_mix(batch):
+    """Requests arrive in bursts while earlier ones are on the device; each
+    result equals the same graph solved alone."""
+    graphs = workloads.batch_graphs(400, seed_base=9000)
+    alone = [batch.solve([g])[0] for g in graphs[:40]]
+    mix = batch.solve(graphs)
+    for a, m in zip(alone, mix[:40]):
+        for key in ("table_hash", "win", "plys", "positions", "base_captures", "won_states", "n_passes"):
+            assert a[key] == m[key]
+    say("server")
+    got = {}
+    with crb.Server(0, hbm_budget=8 << 30, max_batch=64, streams=4) as srv:
+        tickets = []
+        for lo in range(0, len(graphs), 50):
+            for adj, k in graphs[lo:lo + 50]:
+                tickets.append(srv.submit(adj, k))
+            for r in srv.poll(wait_us=200):
+                got[r["ticket"]] = r
+        deadline = time.time() + 60
+        while len(got) < len(graphs) and time.time() < deadline:
+            for r in srv.poll(wait_us=2000):
+                got[r["ticket"]] = r
+        stats = srv.stats()
+    assert len(got) == len(graphs)
+    assert stats["submitted"] == stats["finished"] == len(graphs) and stats["launches"] >= 2
+    for t, m in zip(tickets, mix):
+        r = got[t]
+        assert r["status"] == 0
+        for key in ("table_hash", "win", "plys", "positions", "base_captures", "won_states", "n_passes"):
+            assert r[key] == (m[key] if key != "positions" else m[key] + (0,) * (len(r[key]) - len(m[key])))
+        assert r["latency_ms"] > 0
+
+
+def request_bytes(n, k):
+    """A request's admitted footprint: its state table plus its work planes
+    (won, frontier, next x 2 columns, factorised-pull intermediates, depth bytes)."""
+    from math import comb
+    C = comb(n + k - 1, k)
+    hmax = max([comb(n + i - 1, i) * comb(n + k - i - 1, k - i) for i in range(1, k)], default=0)
+    return 2 * C * n + (5 * C + 2 * hmax + (C + 7) // 8) * 8
+
+
+def test_server_budget_admits_one_at_a_time_and_rejects_oversize(batch):
+    adj, _ = workloads.er_graph(30, 0.15, 4)
+    graphs = [(adj, 3)] * 12
+    ref = batch.solve(graphs[:1])[0]
+    need = request_bytes(30, 3)
+    say("server budget", need)
+    got = {}
+    # room for one request and not two: one graph per launch, one launch at a time
+    with crb.Server(0, hbm_budget=int(need * 1.5), max_batch=64, streams=2) as srv:
+        tickets = [srv.submit(a, k) for a, k in graphs]
+        big, _ = workloads.er_graph(60, 0.2, 3)
+        with pytest.raises(cra.SolverError) as exc:
+            srv.submit(big, 4)
+        assert exc.value.code == -8
+        deadline = time.time() + 60
+        while len(got) < len(graphs) and time.time() < deadline:
+            for r in srv.poll(wait_us=2000):
+                got[r["ticket"]] = r
+        stats = srv.stats()
+    assert stats["launches"] == len(graphs)
+    assert need <= stats["peak_bytes"] <= need * 1.5
+    for t in tickets:
+        assert got[t]["table_hash"] == ref["table_hash"] and got[t]["plys"] == ref["plys"]
