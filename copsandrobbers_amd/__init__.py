@@ -38,7 +38,8 @@ EXPORTS = (
     "cr_get_info", "cr_get_stats", "cr_copy_states", "cr_copy_configs", "cr_level_counts",
     "cr_count_retro_edges", "cr_work_profile", "cr_get_macro_stats", "cr_count_transitions",
     "cr_set_tuning", "cr_set_kernel_timing", "cr_kernel_times", "cr_cache_key",
-    "cr_batch_create", "cr_batch_destroy", "cr_batch_solve", "cr_batch_pass_counts", "cr_batch_last_timing",
+    "cr_batch_create", "cr_batch_destroy", "cr_batch_solve", "cr_batch_set_tuning", "cr_batch_pass_counts",
+    "cr_batch_last_timing", "cr_server_set_tuning",
     "cr_server_create", "cr_server_submit", "cr_server_submit_many", "cr_server_poll", "cr_server_stats", "cr_server_destroy",
     "cr_clock_ns",
 )

@@ -12,6 +12,7 @@ from . import MAX_K, SolverError, _check, _u8, lib
 
 MAX_N = 64
 MAX_BATCH_K = 4
+TUNING = {"threads": 1, "blocks_per_cu": 2}
 
 
 class GraphResult(ctypes.Structure):
@@ -39,6 +40,8 @@ def _lib():
         L.cr_batch_solve.argtypes = [vp, i32, i32p, i32p, ctypes.POINTER(ctypes.c_uint8), u64p,
                                      ctypes.POINTER(GraphResult), ctypes.POINTER(ctypes.c_uint8), u64p]
         L.cr_batch_pass_counts.argtypes = [vp, i32, u64p, i32, i32p]
+        L.cr_batch_set_tuning.argtypes = [vp, i32, ctypes.c_int64]
+        L.cr_server_set_tuning.argtypes = [vp, i32, ctypes.c_int64]
         L.cr_batch_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i32p]
         L.cr_server_create.argtypes = [ctypes.c_int, u64, i32, i32, ctypes.POINTER(vp)]
         L.cr_server_submit.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_uint8), u64, u64p]
@@ -136,6 +139,11 @@ class Batch:
             return out, tlist
         return out
 
+    def set_tuning(self, **knobs):
+        """Launch shape (results never change): threads (0, 256, 512, 1024), blocks_per_cu."""
+        for name, value in knobs.items():
+            _check(_lib().cr_batch_set_tuning(self._h, TUNING[name], int(value)))
+
     def pass_counts(self, index):
         buf = np.zeros(4096, np.uint64)
         n = ctypes.c_int32()
@@ -174,6 +182,10 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+    def set_tuning(self, **knobs):
+        for name, value in knobs.items():
+            _check(_lib().cr_server_set_tuning(self._h, TUNING[name], int(value)))
 
     def submit(self, adj, k, arrival_ns=0):
         adj = np.ascontiguousarray(adj, np.uint8)

@@ -64,25 +64,30 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     return x;
 }
 
+constexpr int kMaxBlockWaves = 16;  // workgroups of up to 1024 threads
+constexpr int kMlp = 8;             // row loads kept in flight per thread in the pull
+
 // Block-wide sums with the result in every thread.
 __device__ __forceinline__ void block_sum_all(unsigned long long& x, unsigned long long& y,
-                                              unsigned long long* red /* [2][kWavesPerBlock] */) {
+                                              unsigned long long* red /* [2][kMaxBlockWaves] */) {
+    const int nw = blockDim.x >> 6;
     x = wave_sum(x);
     y = wave_sum(y);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) {
         red[threadIdx.x >> 6] = x;
-        red[kWavesPerBlock + (threadIdx.x >> 6)] = y;
+        red[kMaxBlockWaves + (threadIdx.x >> 6)] = y;
     }
     __syncthreads();
     x = y = 0;
-    for (int i = 0; i < kWavesPerBlock; ++i) {
+    for (int i = 0; i < nw; ++i) {
         x += red[i];
-        y += red[kWavesPerBlock + i];
+        y += red[kMaxBlockWaves + i];
     }
 }
 
 __device__ __forceinline__ unsigned long long block_min_all(unsigned long long x, unsigned long long* red) {
+    const int nw = blockDim.x >> 6;
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(x, off);
         x = o < x ? o : x;
@@ -91,7 +96,7 @@ __device__ __forceinline__ unsigned long long block_min_all(unsigned long long x
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
     __syncthreads();
     x = ~0ull;
-    for (int i = 0; i < kWavesPerBlock; ++i) x = red[i] < x ? red[i] : x;
+    for (int i = 0; i < nw; ++i) x = red[i] < x ? red[i] : x;
     return x;
 }
 
@@ -168,23 +173,36 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
                     uint32_t A[K], B[K + 1];
                     unrank_j(ra, i, n, s_binom, Mimg, A);
                     unrank_j(rb, K - i, n, s_binom, Mimg, B);
-                    const uint32_t ra_in = rank_j(A, i - 1, n, s_binom, Mimg);
+                    const uint64_t row_in = static_cast<uint64_t>(rank_j(A, i - 1, n, s_binom, Mimg)) * cb_in;
                     uint64_t acc = 0, bits = s_mask[A[i - 1]];
                     while (bits) {
-                        const uint32_t v = static_cast<uint32_t>(__builtin_ctzll(bits));
-                        bits &= bits - 1;
-                        uint32_t M2[K + 1];
-                        int pos = 0;
-                        bool placed = false;
-                        for (int t = 0; t < K - i; ++t) {
-                            if (!placed && v <= B[t]) {
-                                M2[pos++] = v;
-                                placed = true;
+                        // ranks of up to kMlp successor rows first, then their loads together
+                        uint32_t idx[kMlp];
+                        int m = 0;
+#pragma unroll
+                        for (int u = 0; u < kMlp; ++u) {
+                            if (bits == 0) break;
+                            const uint32_t v = static_cast<uint32_t>(__builtin_ctzll(bits));
+                            bits &= bits - 1;
+                            uint32_t M2[K + 1];
+                            int pos = 0;
+                            bool placed = false;
+                            for (int t = 0; t < K - i; ++t) {
+                                if (!placed && v <= B[t]) {
+                                    M2[pos++] = v;
+                                    placed = true;
+                                }
+                                M2[pos++] = B[t];
                             }
-                            M2[pos++] = B[t];
+                            if (!placed) M2[pos++] = v;
+                            idx[u] = rank_j(M2, K - i + 1, n, s_binom, Mimg);
+                            m = u + 1;
                         }
-                        if (!placed) M2[pos++] = v;
-                        acc |= in[static_cast<uint64_t>(ra_in) * cb_in + rank_j(M2, K - i + 1, n, s_binom, Mimg)];
+                        uint64_t got[kMlp];
+#pragma unroll
+                        for (int u = 0; u < kMlp; ++u) got[u] = u < m ? in[row_in + idx[u]] : 0ull;
+#pragma unroll
+                        for (int u = 0; u < kMlp; ++u) acc |= got[u];
                     }
                     dst[o] = i == K ? (acc & ~won0[o] & full) : acc;
                 }
@@ -267,14 +285,15 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_solve(const BatchGraph* __restrict__ graphs,
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restrict__ graphs,
                                                         const uint32_t* __restrict__ order, uint32_t count,
                                                         const uint64_t* __restrict__ masks, uint8_t* tables,
                                                         BatchOut* out, uint64_t* scratch, uint64_t slot_words,
                                                         unsigned int* next) {
     __shared__ uint64_t s_mask[kMaxBN];
     __shared__ uint32_t s_binom[(kMaxBK + 1) * (kBinomCols + 1)];
-    __shared__ unsigned long long s_red[2 * kWavesPerBlock];
+    __shared__ unsigned long long s_red[2 * kMaxBlockWaves];
     __shared__ uint32_t s_item;
     uint64_t* slot = scratch + static_cast<uint64_t>(blockIdx.x) * slot_words;
     for (;;) {
@@ -397,7 +416,11 @@ struct Engine {
     std::vector<uint32_t> index;  // launched graph -> caller's index
     std::vector<BatchGraph> graphs;
     uint64_t total_table_bytes = 0, footprint = 0;
-    int max_grid = 0;
+    int cus = 1;
+    int32_t threads = 0;        // threads per workgroup: 256, 512, 1024, or 0 = by the largest graph
+    int32_t blocks_per_cu = 0;  // resident workgroups per CU (0 = 2048 threads per CU)
+    uint32_t block = kBlock;    // the last launch's workgroup size
+    int32_t share = 1;          // launches in flight together (server slots): each gets 1/share of the CUs
 
     int init(int dev) {
         device = dev;
@@ -407,7 +430,7 @@ struct Engine {
         HIP_TRY(hipEventCreate(&ev1));
         hipDeviceProp_t prop{};
         HIP_TRY(hipGetDeviceProperties(&prop, dev));
-        max_grid = std::max(1, prop.multiProcessorCount) * 8;
+        cus = std::max(1, prop.multiProcessorCount);
         return 0;
     }
     void release() {
@@ -440,11 +463,17 @@ struct Engine {
         std::vector<uint32_t> order(count);
         std::iota(order.begin(), order.end(), 0u);
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return items[a]->cost > items[b]->cost; });
+        // workgroup size: a graph is solved by one workgroup, so large graphs get more waves
+        uint64_t max_c = 0;
+        for (uint32_t i = 0; i < count; ++i) max_c = std::max<uint64_t>(max_c, items[i]->C);
+        block = threads ? static_cast<uint32_t>(threads) : (max_c >= 4096 ? 1024u : 256u);
+        const uint64_t per_cu = blocks_per_cu ? static_cast<uint64_t>(blocks_per_cu) : 2048u / block;
+        const uint64_t max_grid = std::max<uint64_t>(1, static_cast<uint64_t>(cus) * per_cu / share);
         // workgroups: enough to fill the device, bounded by scratch memory
         size_t freeb = 0, totalb = 0;
         HIP_TRY(hipMemGetInfo(&freeb, &totalb));
         const uint64_t avail = freeb > table_off + (1ull << 30) ? (freeb - table_off - (1ull << 30)) / 2 : 0;
-        uint64_t g = std::min<uint64_t>(count, static_cast<uint64_t>(max_grid));
+        uint64_t g = std::min<uint64_t>(count, max_grid);
         g = std::min<uint64_t>(g, std::max<uint64_t>(1, avail / (slot_words * 8)));
         grid = static_cast<uint32_t>(std::max<uint64_t>(g, count ? 1 : 0));
         int rc = 0;
@@ -467,9 +496,14 @@ struct Engine {
         HIP_TRY(hipMemcpyAsync(d_masks.p, h_masks.p, mask_off * 8, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemsetAsync(d_next.p, 0, sizeof(unsigned int), stream));
         HIP_TRY(hipEventRecord(ev0, stream));
-        hipLaunchKernelGGL(k_batch_solve, dim3(grid), dim3(kBlock), 0, stream, d_graphs.as<BatchGraph>(),
-                           d_order.as<uint32_t>(), count, d_masks.as<uint64_t>(), d_tables.as<uint8_t>(),
-                           d_out.as<BatchOut>(), d_scratch.as<uint64_t>(), slot_words, d_next.as<unsigned int>());
+        auto args = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, d_graphs.as<BatchGraph>(),
+                               d_order.as<uint32_t>(), count, d_masks.as<uint64_t>(), d_tables.as<uint8_t>(),
+                               d_out.as<BatchOut>(), d_scratch.as<uint64_t>(), slot_words, d_next.as<unsigned int>());
+        };
+        if (block == 1024) args(k_batch_solve<1024>);
+        else if (block == 512) args(k_batch_solve<512>);
+        else args(k_batch_solve<256>);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ev1, stream));
         HIP_TRY(hipMemcpyAsync(h_out.p, d_out.p, count * sizeof(BatchOut), hipMemcpyDeviceToHost, stream));
@@ -702,6 +736,38 @@ int cr_batch_solve(cr_batch* b, int32_t count, const int32_t* n, const int32_t* 
     return CR_OK;
 }
 
+namespace {
+int set_engine_tuning(Engine& e, int32_t key, int64_t value) {
+    switch (key) {
+        case CR_BATCH_TUNE_THREADS:
+            if (value != 0 && value != 256 && value != 512 && value != 1024)
+                return fail(CR_ERR_ARG, "threads per workgroup must be 0, 256, 512 or 1024");
+            e.threads = static_cast<int32_t>(value);
+            return CR_OK;
+        case CR_BATCH_TUNE_BLOCKS_PER_CU:
+            if (value < 0 || value > 32) return fail(CR_ERR_ARG, "workgroups per CU must be 0..32");
+            e.blocks_per_cu = static_cast<int32_t>(value);
+            return CR_OK;
+        default: return fail(CR_ERR_ARG, "unknown batch tuning key %d", key);
+    }
+}
+}  // namespace
+
+int cr_batch_set_tuning(cr_batch* b, int32_t key, int64_t value) {
+    if (b == nullptr) return fail(CR_ERR_ARG, "null batch");
+    return set_engine_tuning(b->eng, key, value);
+}
+
+int cr_server_set_tuning(cr_server* s, int32_t key, int64_t value) {
+    if (s == nullptr) return fail(CR_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto& sl : s->slots) {
+        if (sl->busy) return fail(CR_ERR_ORDER, "tune the server before submitting");
+        if (int rc = set_engine_tuning(sl->eng, key, value)) return rc;
+    }
+    return CR_OK;
+}
+
 int cr_batch_pass_counts(cr_batch* b, int32_t index, uint64_t* out, int32_t cap, int32_t* n_passes) {
     if (b == nullptr) return fail(CR_ERR_ARG, "null batch");
     if (index < 0 || index >= static_cast<int32_t>(b->last.size()) || !b->valid[index])
@@ -736,6 +802,7 @@ int cr_server_create(int device, uint64_t hbm_budget, int32_t max_batch, int32_t
     for (int i = 0; i < streams; ++i) {
         auto sl = std::make_unique<cr_server::Slot>();
         if (int rc = sl->eng.init(device)) return rc;
+        sl->eng.share = streams;  // concurrent launches split the device instead of queueing behind each other
         s->slots.push_back(std::move(sl));
     }
     cr_server* raw = s.get();

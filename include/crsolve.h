@@ -230,6 +230,13 @@ void cr_batch_destroy(cr_batch* b);
 int cr_batch_solve(cr_batch* b, int32_t count, const int32_t* n, const int32_t* k, const uint8_t* adj,
                    const uint64_t* adj_offset, cr_graph_result* results, uint8_t* tables,
                    const uint64_t* table_offset);
+/* Launch shape of cr_batch_solve and of the server's launches (speed only). */
+enum cr_batch_tuning {
+    CR_BATCH_TUNE_THREADS = 1,      /* threads per workgroup (one graph at a time): 256, 512, 1024; 0 = 1024
+                                       when the batch's largest graph has >= 4096 configurations, else 256 */
+    CR_BATCH_TUNE_BLOCKS_PER_CU = 2 /* persistent workgroups per CU; 0 = 2048 threads per CU */
+};
+int cr_batch_set_tuning(cr_batch* b, int32_t key, int64_t value);
 /* Per-pass "Found n" counts of graph `index` of the last cr_batch_solve. */
 int cr_batch_pass_counts(cr_batch* b, int32_t index, uint64_t* out, int32_t cap, int32_t* n_passes);
 /* Device time of the last cr_batch_solve (launch to last result) and its grid. */
@@ -253,6 +260,8 @@ int cr_server_submit(cr_server* s, int32_t n, int32_t k, const uint8_t* adj, uin
 int cr_server_submit_many(cr_server* s, int32_t count, const int32_t* n, const int32_t* k, const uint8_t* adj,
                           const uint64_t* adj_offset, const uint64_t* arrival_ns, uint64_t* tickets,
                           int32_t* status);
+/* cr_batch_set_tuning for every launch slot (before the first submit). */
+int cr_server_set_tuning(cr_server* s, int32_t key, int64_t value);
 /* Moves up to cap finished results to out; waits up to wait_us for the first. */
 int cr_server_poll(cr_server* s, cr_graph_result* out, int32_t cap, int32_t wait_us, int32_t* count);
 /* Requests submitted, finished, launches made, peak bytes admitted at once. */
