@@ -33,10 +33,12 @@ STATUS = {
 # Names every build of include/crsolve.h exports.
 EXPORTS = (
     "cr_solver_create", "cr_solver_destroy", "cr_last_error", "cr_parse_matrix",
-    "cr_load_graph_file", "cr_load_graph_matrix", "cr_set_variant", "cr_build_aux_graph",
+    "cr_load_graph_file", "cr_load_graph_matrix", "cr_set_variant", "cr_set_rules", "cr_parse_edge_list",
+    "cr_load_edge_file", "cr_build_aux_graph",
     "cr_initialize_captures", "cr_main_loop", "cr_find_final_result", "cr_save_aux_graph",
     "cr_get_info", "cr_get_stats", "cr_copy_states", "cr_copy_configs", "cr_level_counts",
-    "cr_count_retro_edges", "cr_work_profile", "cr_get_macro_stats", "cr_count_transitions",
+    "cr_count_retro_edges", "cr_work_profile", "cr_get_macro_stats", "cr_write_dp_table", "cr_extract_game",
+    "cr_write_perfect_game", "cr_cop_number", "cr_count_transitions",
     "cr_set_tuning", "cr_set_kernel_timing", "cr_kernel_times", "cr_cache_key",
     "cr_batch_create", "cr_batch_destroy", "cr_batch_solve", "cr_batch_set_tuning", "cr_batch_pass_counts",
     "cr_batch_last_timing", "cr_server_set_tuning",
@@ -61,6 +63,14 @@ class Info(ctypes.Structure):
 class Verdict(ctypes.Structure):
     _fields_ = [("win", ctypes.c_int32), ("plys", ctypes.c_int32), ("best_cid", ctypes.c_uint64),
                 ("positions", ctypes.c_uint8 * MAX_K)]
+
+
+class GameStep(ctypes.Structure):
+    _fields_ = [("cops", ctypes.c_uint8 * MAX_K), ("robber", ctypes.c_int32), ("turn", ctypes.c_int32)]
+
+
+TURNS = ("Cop's Turn", "Robber's Turn", "Game Over - Captured!")
+GAME_ORDER = {"py": 0, "cpp": 1}
 
 
 class Stats(ctypes.Structure):
@@ -104,6 +114,14 @@ def lib():
         L.cr_load_graph_file.argtypes = [vp, ctypes.c_char_p, i32, i32]
         L.cr_load_graph_matrix.argtypes = [vp, i32, u8p, i32, i32]
         L.cr_set_variant.argtypes = [vp, i32]
+        L.cr_set_rules.argtypes = [vp, i32, i32]
+        L.cr_parse_edge_list.argtypes = [u8p, ctypes.c_uint64, u8p, i32]
+        L.cr_parse_edge_list.restype = i32
+        L.cr_load_edge_file.argtypes = [vp, ctypes.c_char_p, i32, i32]
+        L.cr_write_dp_table.argtypes = [vp, ctypes.c_char_p]
+        L.cr_extract_game.argtypes = [vp, i32, ctypes.POINTER(GameStep), i32, ctypes.POINTER(i32)]
+        L.cr_write_perfect_game.argtypes = [vp, i32, ctypes.c_char_p]
+        L.cr_cop_number.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(Verdict)]
         L.cr_build_aux_graph.argtypes = [vp]
         L.cr_initialize_captures.argtypes = [vp, u64p]
         L.cr_main_loop.argtypes = [vp, u64p, i32, ctypes.POINTER(i32)]
@@ -151,6 +169,16 @@ def parse_matrix(text: bytes):
     return n, adj
 
 
+def parse_edge_list(text: bytes):
+    """legacy/util/edge_parser.py: 1-based undirected edge list -> (n, adj uint8[n, n])."""
+    buf = np.frombuffer(bytes(text) or b"\0", np.uint8).copy()
+    n = lib().cr_parse_edge_list(_u8(buf), len(text), None, 0)
+    adj = np.zeros((max(n, 0), max(n, 0)), np.uint8)
+    if n > 0:
+        lib().cr_parse_edge_list(_u8(buf), len(text), _u8(adj), n)
+    return n, adj
+
+
 def cache_key(graph_path, k, p, variant=VARIANT_COLUMNS):
     buf = ctypes.create_string_buffer(4096)
     n = lib().cr_cache_key(str(graph_path).encode(), k, p, variant, buf, len(buf))
@@ -192,6 +220,14 @@ class Solver:
     def load_graph_matrix(self, adj, k, p=1, variant=VARIANT_COLUMNS):
         adj = np.ascontiguousarray(adj, np.uint8)
         _check(lib().cr_load_graph_matrix(self._h, adj.shape[0], _u8(adj), k, p))
+        _check(lib().cr_set_variant(self._h, variant))
+
+    def set_rules(self, cop_stay=False, robber_stay=False):
+        """Stay moves for cops / robber (AuxGraph::setSelfEdges); next build."""
+        _check(lib().cr_set_rules(self._h, int(bool(cop_stay)), int(bool(robber_stay))))
+
+    def load_edge_file(self, filename, k, p=1, variant=VARIANT_COLUMNS):
+        _check(lib().cr_load_edge_file(self._h, str(filename).encode(), k, p))
         _check(lib().cr_set_variant(self._h, variant))
 
     def build_aux_graph(self):
@@ -278,6 +314,33 @@ class Solver:
         out["nodes_per_step"] = [int(x) for x in nodes[:m.steps]]
         out["edges_per_step"] = [int(x) for x in edges[:max(m.steps - 1, 0)]]
         return out
+
+    def write_dp_table(self, path):
+        _check(lib().cr_write_dp_table(self._h, str(path).encode()))
+
+    def extract_game(self, order="py", cap=8192):
+        """The perfect game as [{"cops": [...], "robber": r, "turn": "..."}, ...];
+        order "py" (legacy/python/k_cops.py) or "cpp" (legacy/src/k_cops_rounds.cpp)."""
+        steps = (GameStep * cap)()
+        n = ctypes.c_int32()
+        _check(lib().cr_extract_game(self._h, GAME_ORDER[order], steps, cap, ctypes.byref(n)))
+        k = self.info().k
+        return [{"cops": [int(x) for x in steps[i].cops[:k]], "robber": int(steps[i].robber),
+                 "turn": TURNS[steps[i].turn]} for i in range(min(n.value, cap))]
+
+    def write_perfect_game(self, path, order="py"):
+        _check(lib().cr_write_perfect_game(self._h, GAME_ORDER[order], str(path).encode()))
+
+    def cop_number(self, k_max=4):
+        """Least k <= k_max with a WIN (0 if none) and that solve's verdict."""
+        kk = ctypes.c_int32()
+        v = Verdict()
+        _check(lib().cr_cop_number(self._h, k_max, ctypes.byref(kk), ctypes.byref(v)))
+        verdict = None
+        if v.win:
+            verdict = {"plys": int(v.plys), "best_cid": int(v.best_cid),
+                       "positions": tuple(int(x) for x in v.positions[:max(kk.value, 1)])}
+        return int(kk.value), verdict
 
     def count_transitions(self):
         t = ctypes.c_uint64()

@@ -45,7 +45,7 @@ def build_library():
     tmp = out + ".tmp"
     run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}",
          os.path.join(CSRC, "crsolve.hip"), os.path.join(CSRC, "macro.hip"), os.path.join(CSRC, "batch.hip"),
-         os.path.join(CSRC, "columns.hip"), os.path.join(CSRC, "cr_host.cpp"),
+         os.path.join(CSRC, "columns.hip"), os.path.join(CSRC, "game.hip"), os.path.join(CSRC, "cr_host.cpp"),
          "-o", tmp])
     os.replace(tmp, out)
     return out

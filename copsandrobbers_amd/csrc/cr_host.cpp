@@ -225,4 +225,144 @@ int write_aux_blob(const std::string& path, const Adjacency& g, int32_t k, int32
     return ok ? 0 : CR_ERR_IO;
 }
 
+int32_t parse_edge_list(const uint8_t* text, uint64_t len, uint8_t* adj, int32_t cap) {
+    if (text == nullptr) return 0;
+    std::vector<std::pair<uint64_t, uint64_t>> edges;
+    uint64_t max_id = 0;
+    auto is_space = [](uint8_t c) { return c == ' ' || c == '\t' || c == '\v' || c == '\f'; };
+    uint64_t i = 0;
+    while (i < len) {
+        uint64_t end = i;
+        while (end < len && text[end] != '\n' && text[end] != '\r') ++end;  // universal newlines
+        uint64_t a = i, b = end;
+        while (a < b && is_space(text[a])) ++a;
+        while (b > a && is_space(text[b - 1])) --b;
+        if (a < b && text[a] != '#') {
+            // tokens split on whitespace and commas; only all-digit tokens count
+            uint64_t nums[2];
+            int got = 0;
+            uint64_t t = a;
+            while (t < b && got < 2) {
+                while (t < b && (is_space(text[t]) || text[t] == ',')) ++t;
+                uint64_t e = t;
+                while (e < b && !is_space(text[e]) && text[e] != ',') ++e;
+                if (e > t) {
+                    bool digits = true;
+                    uint64_t v = 0;
+                    for (uint64_t q = t; q < e; ++q) {
+                        if (text[q] < '0' || text[q] > '9') { digits = false; break; }
+                        v = v > 100000000ull ? v : v * 10 + (text[q] - '0');  // saturates far above any graph size
+                    }
+                    if (digits) nums[got++] = v;
+                }
+                t = e;
+            }
+            if (got == 2) {
+                edges.emplace_back(nums[0], nums[1]);
+                max_id = std::max(max_id, std::max(nums[0], nums[1]));
+            }
+        }
+        i = end + 1;
+    }
+    if (edges.empty() || max_id == 0) return 0;
+    if (max_id > 0x7fffffffull) return 0;
+    const int32_t n = static_cast<int32_t>(max_id);
+    if (adj == nullptr || n > cap) return n;
+    std::memset(adj, 0, static_cast<size_t>(n) * n);
+    for (auto [u, v] : edges) {
+        const uint64_t iu = u == 0 ? static_cast<uint64_t>(n - 1) : u - 1;  // Python index -1
+        const uint64_t iv = v == 0 ? static_cast<uint64_t>(n - 1) : v - 1;
+        adj[iu * n + iv] = 1;
+        adj[iv * n + iu] = 1;
+    }
+    return n;
+}
+
+int write_dp_table(const std::string& path, int32_t n, int32_t k, int32_t columns, uint64_t configs,
+                   const std::function<int(uint64_t, uint64_t, uint8_t*)>& fetch_states) {
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return CR_ERR_IO;
+    std::vector<uint8_t> cur(static_cast<size_t>(k), 0);  // lexicographic odometer (AuxGraph.h:155-176)
+    const uint64_t chunk = std::max<uint64_t>(1, (4ull << 20) / (static_cast<uint64_t>(n) * columns));
+    std::vector<uint8_t> rows;
+    std::string out;
+    out.reserve(1 << 22);
+    out.push_back('[');
+    bool first = true;
+    int rc = 0;
+    char buf[64];
+    for (uint64_t c0 = 0; c0 < configs && rc == 0; c0 += chunk) {
+        const uint64_t cnt = std::min(chunk, configs - c0);
+        rows.resize(cnt * n * columns);
+        if (fetch_states(c0 * n * columns, rows.size(), rows.data()) != 0) {
+            rc = CR_ERR_DEVICE;
+            break;
+        }
+        for (uint64_t c = 0; c < cnt; ++c) {
+            std::string cops = "[";
+            for (int32_t i = 0; i < k; ++i) {
+                std::snprintf(buf, sizeof buf, i ? ", %d" : "%d", cur[i]);
+                cops += buf;
+            }
+            cops += "]";
+            for (int32_t r = 0; r < n; ++r) {
+                const uint8_t b = rows[(c * n + r) * columns];
+                const bool win = b & 1;
+                const int steps = win ? ((b >> 1) + 1) / 2 : -1;
+                if (!first) out += ", ";
+                first = false;
+                out += "{\"cops\": ";
+                out += cops;
+                std::snprintf(buf, sizeof buf, ", \"robber\": %d, \"cop_wins\": %s, \"steps_to_win\": %d}", r,
+                              win ? "true" : "false", steps);
+                out += buf;
+            }
+            // next configuration
+            int32_t q = k - 1;
+            while (q >= 0 && cur[q] == n - 1) --q;
+            if (q >= 0) {
+                ++cur[q];
+                for (int32_t i = q + 1; i < k; ++i) cur[i] = cur[q];
+            }
+        }
+        if (out.size() > (1u << 21)) {
+            if (std::fwrite(out.data(), 1, out.size(), f) != out.size()) rc = CR_ERR_IO;
+            out.clear();
+        }
+    }
+    out.push_back(']');
+    if (rc == 0 && std::fwrite(out.data(), 1, out.size(), f) != out.size()) rc = CR_ERR_IO;
+    if (std::fclose(f) != 0 && rc == 0) rc = CR_ERR_IO;
+    return rc;
+}
+
+int write_game_json(const std::string& path, int32_t k, const std::vector<GameStep>& steps) {
+    static const char* turns[] = {"Cop's Turn", "Robber's Turn", "Game Over - Captured!"};
+    std::string out;
+    if (steps.empty()) {
+        out = "[]";
+    } else {
+        out = "[\n";
+        char buf[64];
+        for (size_t i = 0; i < steps.size(); ++i) {
+            const GameStep& s = steps[i];
+            out += "    {\n        \"cops\": [\n";
+            for (int32_t j = 0; j < k; ++j) {
+                std::snprintf(buf, sizeof buf, "            %d%s\n", s.cops[j], j + 1 < k ? "," : "");
+                out += buf;
+            }
+            std::snprintf(buf, sizeof buf, "        ],\n        \"robber\": %d,\n", s.robber);
+            out += buf;
+            out += "        \"turn\": \"";
+            out += turns[s.turn >= 0 && s.turn <= 2 ? s.turn : 0];
+            out += i + 1 < steps.size() ? "\"\n    },\n" : "\"\n    }\n";
+        }
+        out += "]";
+    }
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return CR_ERR_IO;
+    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    return (std::fclose(f) == 0 && ok) ? 0 : CR_ERR_IO;
+}
+
 }  // namespace cr

@@ -48,4 +48,25 @@ int write_aux_blob(const std::string& path, const Adjacency& g, int32_t k, int32
                    bool cop_stay, bool robber_stay, uint64_t num_states,
                    const std::function<int(uint64_t, uint64_t, uint8_t*)>& fetch_states);
 
+// legacy/util/edge_parser.py:4-63: 1-based "u v" / "u, v" lines, '#'
+// comments, undirected; N = the largest id (an id 0 lands on row N-1, as
+// Python's negative index does there). Returns N, 0 when there is no edge.
+int32_t parse_edge_list(const uint8_t* text, uint64_t len, uint8_t* adj, int32_t cap);
+
+// The dp_tables JSON of the legacy round solver (k_cops_rounds.cpp:211-281,
+// k_cops.py:86-131) from the column-0 bytes of a solved table: one object per
+// (cId, r), json.dumps spacing. fetch_states(offset, count, dst) reads table
+// bytes. Returns 0 or a negative CR_ERR_* status.
+int write_dp_table(const std::string& path, int32_t n, int32_t k, int32_t columns, uint64_t configs,
+                   const std::function<int(uint64_t, uint64_t, uint8_t*)>& fetch_states);
+
+// One step of an extracted game: cops, robber, turn (0 cop, 1 robber, 2 captured).
+struct GameStep {
+    uint8_t cops[16];
+    int32_t robber;
+    int32_t turn;
+};
+// json.dump(game, indent=4) of k_cops.py:243-260 / export_helper.py:47-49.
+int write_game_json(const std::string& path, int32_t k, const std::vector<GameStep>& steps);
+
 }  // namespace cr

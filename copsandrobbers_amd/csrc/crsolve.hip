@@ -48,6 +48,7 @@
 
 #include "cr_device.h"
 #include "cr_columns.h"
+#include "cr_game.h"
 #include "cr_host.h"
 #include "cr_macro.h"
 #include "crsolve.h"
@@ -899,6 +900,9 @@ struct cr_solver {
     hipStream_t stream = nullptr;
     Stage stage = kEmpty;
     cr::Adjacency g;
+    cr::Adjacency gc;              // cop moves: g, plus each vertex itself when cops may stay
+    std::vector<uint64_t> rmask;   // [n][4] robber moves: out-edges, plus r itself when the robber may stay
+    int32_t cop_stay = 0, robber_stay = 0;  // AuxGraph::setSelfEdges (AuxGraph.h:81-91)
     std::string path;
     int32_t k = 0, p = 1, variant = CR_VARIANT_COLUMNS;
     uint64_t C = 0, S = 0;
@@ -1249,11 +1253,11 @@ int upload_graph(cr_solver* s) {
         const uint16_t o = static_cast<uint16_t>(acc);
         img.push_back(static_cast<uint8_t>(o & 0xff));
         img.push_back(static_cast<uint8_t>(o >> 8));
-        if (v < n) acc += static_cast<uint32_t>(s->g.in[v].size());
+        if (v < n) acc += static_cast<uint32_t>(s->gc.in[v].size());
     }
     align4();
     const uint32_t off_in_nbr = static_cast<uint32_t>(img.size());
-    for (int v = 0; v < n; ++v) img.insert(img.end(), s->g.in[v].begin(), s->g.in[v].end());
+    for (int v = 0; v < n; ++v) img.insert(img.end(), s->gc.in[v].begin(), s->gc.in[v].end());
     align4();
     const uint32_t off_out_off = static_cast<uint32_t>(img.size());
     acc = 0;
@@ -1261,20 +1265,18 @@ int upload_graph(cr_solver* s) {
         const uint16_t o = static_cast<uint16_t>(acc);
         img.push_back(static_cast<uint8_t>(o & 0xff));
         img.push_back(static_cast<uint8_t>(o >> 8));
-        if (v < n) acc += static_cast<uint32_t>(s->g.out[v].size());
+        if (v < n) acc += static_cast<uint32_t>(s->gc.out[v].size());
     }
     align4();
     const uint32_t off_out_nbr = static_cast<uint32_t>(img.size());
-    for (int v = 0; v < n; ++v) img.insert(img.end(), s->g.out[v].begin(), s->g.out[v].end());
+    for (int v = 0; v < n; ++v) img.insert(img.end(), s->gc.out[v].begin(), s->gc.out[v].end());
     align4();
     if (img.size() > 150 * 1024) return fail(CR_ERR_UNSUPPORTED, "graph image of %zu bytes exceeds LDS", img.size());
     int rc = 0;
     s->d_image = dev_alloc<uint8_t>(img.size(), s->device_bytes, rc);
     if (!s->d_image) return rc;
     HIP_TRY(hipMemcpy(s->d_image, img.data(), img.size(), hipMemcpyHostToDevice));
-    std::vector<uint64_t> om(static_cast<size_t>(n) * 4, 0);
-    for (int u = 0; u < n; ++u)
-        for (uint8_t v : s->g.out[u]) om[u * 4 + (v >> 6)] |= 1ull << (v & 63);
+    const std::vector<uint64_t>& om = s->rmask;
     s->d_outmask = dev_alloc<uint64_t>(om.size(), s->device_bytes, rc);
     if (!s->d_outmask) return rc;
     HIP_TRY(hipMemcpy(s->d_outmask, om.data(), om.size() * 8, hipMemcpyHostToDevice));
@@ -1301,9 +1303,10 @@ int do_load(cr_solver* s, int32_t n, const uint8_t* adj, int32_t k, int32_t p) {
     if (k <= 0 || k > CR_MAX_K) return fail(CR_ERR_K_RANGE, "k=%d outside 1..%d", k, CR_MAX_K);
     if (p < 1) return fail(CR_ERR_P_RANGE, "p=%d must be >= 1", p);
     s->g.build(n, adj);
-    for (int v = 0; v < n; ++v)
-        if (s->g.out[v].empty())
-            return fail(CR_ERR_ISOLATED_VERTEX, "vertex %d has no move; the reference's move generation is undefined there", v);
+    if (!s->cop_stay)  // with stay moves every cop has a move
+        for (int v = 0; v < n; ++v)
+            if (s->g.out[v].empty())
+                return fail(CR_ERR_ISOLATED_VERTEX, "vertex %d has no move; the reference's move generation is undefined there", v);
     s->k = k;
     s->p = p;
     s->stage = kLoaded;
@@ -1322,7 +1325,7 @@ void set_pull_threshold(cr_solver* s) {
     }
     const int n = s->g.n, K = s->k;
     uint64_t edges = 0;
-    for (int v = 0; v < n; ++v) edges += s->g.out[v].size();
+    for (int v = 0; v < n; ++v) edges += s->gc.out[v].size();
     const double deg = static_cast<double>(edges) / n;
     double pull_reads = 0;
     for (int i = 1; i <= K; ++i) pull_reads += static_cast<double>(s->ms_count[i]) * s->ms_count[K - i] * deg;
@@ -1359,6 +1362,8 @@ int alloc_pull(cr_solver* s) {
 // and the cop masks; the trajectory DAG is built by the main loop.
 int build_macro(cr_solver* s) {
     const int n = s->g.n;
+    if (s->cop_stay || s->robber_stay)
+        return fail(CR_ERR_UNSUPPORTED, "stay moves are not part of the limited-visibility rules (the robber must move, k_cops_visibility_v2.cpp:226)");
     s->columns = 1;
     s->S = s->C * static_cast<uint64_t>(n);
     s->W = (n + 63) / 64;
@@ -1403,7 +1408,7 @@ int build_columns(cr_solver* s) {
             uint64_t nxt[4] = {0, 0, 0, 0};
             for (int v = 0; v < n; ++v)
                 if ((cur[v >> 6] >> (v & 63)) & 1)
-                    for (uint8_t u : s->g.out[v]) nxt[u >> 6] |= 1ull << (u & 63);
+                    for (int w = 0; w < 4; ++w) nxt[w] |= s->rmask[v * 4 + w];  // robber moves (+ stay)
             std::memcpy(cur, nxt, sizeof cur);
         }
         std::memcpy(&s->reach[static_cast<size_t>(r) * 4], cur, sizeof cur);
@@ -1412,17 +1417,38 @@ int build_columns(cr_solver* s) {
     return 0;
 }
 
+// Move sets under the solver's rules: cop moves add the cop's own vertex when
+// cops may stay (AuxGraph.h:203-205), robber moves add r when the robber may
+// stay (k_cops_visibility.cpp:253-255).
+int derive_moves(cr_solver* s) {
+    const int n = s->g.n;
+    std::vector<uint8_t> m = s->g.matrix;
+    if (s->cop_stay)
+        for (int v = 0; v < n; ++v) m[static_cast<size_t>(v) * n + v] = 1;
+    s->gc.build(n, m.data());
+    for (int v = 0; v < n; ++v)
+        if (s->gc.out[v].empty())
+            return fail(CR_ERR_ISOLATED_VERTEX, "vertex %d has no move; the reference's move generation is undefined there", v);
+    s->rmask.assign(static_cast<size_t>(n) * 4, 0);
+    for (int u = 0; u < n; ++u) {
+        for (uint8_t v : s->g.out[u]) s->rmask[u * 4 + (v >> 6)] |= 1ull << (v & 63);
+        if (s->robber_stay) s->rmask[u * 4 + (u >> 6)] |= 1ull << (u & 63);
+    }
+    return 0;
+}
+
 int do_build(cr_solver* s) {
     if (s->stage < kLoaded) return fail(CR_ERR_ORDER, "buildAuxGraph before loadGraphFile");
     s->release();
     const int n = s->g.n;
+    if (int rc = derive_moves(s)) return rc;
     s->C = cr::config_count(n, s->k);
     if (s->C == 0) return fail(CR_ERR_K_RANGE, "no configuration for k=%d", s->k);
     if (s->C >= 0xffffffffull) return fail(CR_ERR_TOO_LARGE, "%llu configurations exceed 32-bit ids", static_cast<unsigned long long>(s->C));
     uint64_t prod = 1, prod_out = 1;
     for (int i = 0; i < s->k; ++i) {
-        prod *= static_cast<uint64_t>(std::max(1, s->g.max_in));
-        prod_out *= static_cast<uint64_t>(std::max(1, s->g.max_out));
+        prod *= static_cast<uint64_t>(std::max(1, s->gc.max_in));
+        prod_out *= static_cast<uint64_t>(std::max(1, s->gc.max_out));
     }
     if (prod >= (1ull << 32) || prod_out >= (1ull << 32))
         return fail(CR_ERR_TOO_LARGE, "team-move product %llu too large", static_cast<unsigned long long>(std::max(prod, prod_out)));
@@ -1682,7 +1708,7 @@ int do_loop(cr_solver* s) {
 int count_moves(cr_solver* s, int32_t mode, unsigned long long* r, MoveArgs mv = MoveArgs{}) {
     HIP_TRY(hipSetDevice(s->device));
     uint64_t prod = 1;
-    const int deg = std::max(1, mode == 1 ? s->g.max_out : s->g.max_in);
+    const int deg = std::max(1, mode == 1 ? s->gc.max_out : s->gc.max_in);
     for (int i = 0; i < s->k; ++i) prod *= static_cast<uint64_t>(deg);
     uint32_t slots = 64;
     while (slots < 2 * prod) slots <<= 1;
@@ -1823,11 +1849,42 @@ int cr_load_graph_file(cr_solver* s, const char* filename, int32_t k, int32_t p)
     return do_load(s, n, adj.data(), k, p);
 }
 
+int32_t cr_parse_edge_list(const uint8_t* text, uint64_t len, uint8_t* adj, int32_t cap) {
+    return cr::parse_edge_list(text, len, adj, cap);
+}
+
+int cr_load_edge_file(cr_solver* s, const char* filename, int32_t k, int32_t p) {
+    if (int rc = check_solver(s)) return rc;
+    if (filename == nullptr) return fail(CR_ERR_ARG, "null filename");
+    std::FILE* f = std::fopen(filename, "rb");
+    if (!f) return fail(CR_ERR_EMPTY_GRAPH, "Error: The file '%s' was not found.", filename);
+    std::vector<uint8_t> text;
+    uint8_t buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) text.insert(text.end(), buf, buf + got);
+    std::fclose(f);
+    const int32_t n = cr::parse_edge_list(text.data(), text.size(), nullptr, 0);
+    if (n <= 0) return fail(CR_ERR_EMPTY_GRAPH, "Error: No valid edges found in file %s.", filename);
+    if (n > CR_MAX_N) return fail(CR_ERR_GRAPH_LIMIT, "graph has %d vertices; the format limit is %d", n, CR_MAX_N);
+    std::vector<uint8_t> adj(static_cast<size_t>(n) * n);
+    cr::parse_edge_list(text.data(), text.size(), adj.data(), n);
+    s->path = filename;
+    return do_load(s, n, adj.data(), k, p);
+}
+
 int cr_load_graph_matrix(cr_solver* s, int32_t n, const uint8_t* adj, int32_t k, int32_t p) {
     if (int rc = check_solver(s)) return rc;
     if (adj == nullptr && n > 0) return fail(CR_ERR_ARG, "null matrix");
     s->path = "matrix";
     return do_load(s, n, adj, k, p);
+}
+
+int cr_set_rules(cr_solver* s, int32_t cop_stay, int32_t robber_stay) {
+    if (int rc = check_solver(s)) return rc;
+    s->cop_stay = cop_stay ? 1 : 0;
+    s->robber_stay = robber_stay ? 1 : 0;
+    if (s->stage > kLoaded) s->stage = kLoaded;
+    return 0;
 }
 
 int cr_set_variant(cr_solver* s, int32_t variant) {
@@ -1898,7 +1955,7 @@ int cr_save_aux_graph(cr_solver* s, const char* path) {
         if (hipMemcpyAsync(dst, s->d_table + off, cnt, hipMemcpyDeviceToHost, s->stream) != hipSuccess) return -1;
         return hipStreamSynchronize(s->stream) == hipSuccess ? 0 : -1;
     };
-    const int rc = cr::write_aux_blob(path, s->g, s->k, s->columns, false, false, s->S, fetch);
+    const int rc = cr::write_aux_blob(path, s->g, s->k, s->columns, s->cop_stay != 0, s->robber_stay != 0, s->S, fetch);
     if (rc != 0) return fail(rc, "could not write %s", path);
     return 0;
 }
@@ -1960,6 +2017,8 @@ int cr_level_counts(cr_solver* s, uint64_t* out) {
 
 int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_t* won) {
     if (int rc = check_solver(s)) return rc;
+    if (s->cop_stay != s->robber_stay)
+        return fail(CR_ERR_UNSUPPORTED, "retro-edges are counted for equal cop and robber move rules");
     if (s->variant == CR_VARIANT_MACRO || s->p != 1)
         return fail(CR_ERR_UNSUPPORTED, "depth levels and retro-edges are kept for the p = 1 column solver");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
@@ -1973,6 +2032,8 @@ int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_
 
 int cr_work_profile(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint8_t* pulled) {
     if (int rc = check_solver(s)) return rc;
+    if (s->cop_stay != s->robber_stay)
+        return fail(CR_ERR_UNSUPPORTED, "retro-edges are counted for equal cop and robber move rules");
     if (s->variant == CR_VARIANT_MACRO || s->p != 1)
         return fail(CR_ERR_UNSUPPORTED, "depth levels and retro-edges are kept for the p = 1 column solver");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
@@ -2008,6 +2069,86 @@ int cr_get_macro_stats(cr_solver* s, cr_macro_stats* out, uint64_t* nodes_per_st
     for (int32_t d = 0; d < cap && d < out->steps; ++d) {
         if (nodes_per_step) nodes_per_step[d] = m.nodes[d];
         if (edges_per_step) edges_per_step[d] = d < static_cast<int32_t>(m.edges.size()) ? m.edges[d] : 0;
+    }
+    return 0;
+}
+
+namespace {
+int check_round_table(cr_solver* s) {
+    if (int rc = check_solver(s)) return rc;
+    if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    if (s->variant != CR_VARIANT_COLUMNS || s->p != 1)
+        return fail(CR_ERR_UNSUPPORTED, "rounds are defined on the two-column table (column variant, p = 1)");
+    return 0;
+}
+}  // namespace
+
+int cr_write_dp_table(cr_solver* s, const char* path) {
+    if (int rc = check_round_table(s)) return rc;
+    if (path == nullptr) return fail(CR_ERR_ARG, "null path");
+    HIP_TRY(hipSetDevice(s->device));
+    auto fetch = [s](uint64_t off, uint64_t cnt, uint8_t* dst) -> int {
+        if (hipMemcpyAsync(dst, s->d_table + off, cnt, hipMemcpyDeviceToHost, s->stream) != hipSuccess) return -1;
+        return hipStreamSynchronize(s->stream) == hipSuccess ? 0 : -1;
+    };
+    const int rc = cr::write_dp_table(path, s->g.n, s->k, s->columns, s->C, fetch);
+    if (rc != 0) return fail(rc, "could not write %s", path);
+    return 0;
+}
+
+int cr_extract_game(cr_solver* s, int32_t order, cr_game_step* out, int32_t cap, int32_t* length) {
+    if (int rc = check_round_table(s)) return rc;
+    if (cap > 0 && out == nullptr) return fail(CR_ERR_ARG, "null output");
+    if (order != CR_GAME_ORDER_PY && order != CR_GAME_ORDER_CPP) return fail(CR_ERR_ARG, "game order %d", order);
+    HIP_TRY(hipSetDevice(s->device));
+    std::vector<cr::GameStep> steps;
+    if (int rc = crd::game_extract(s->gv, s->k, s->columns, s->d_table, s->g.matrix, s->cop_stay != 0,
+                                   s->robber_stay != 0, order == CR_GAME_ORDER_CPP, s->stream, steps))
+        return rc;
+    for (int32_t i = 0; i < cap && i < static_cast<int32_t>(steps.size()); ++i) {
+        std::memset(&out[i], 0, sizeof out[i]);
+        std::memcpy(out[i].cops, steps[i].cops, CR_MAX_K);
+        out[i].robber = steps[i].robber;
+        out[i].turn = steps[i].turn;
+    }
+    if (length) *length = static_cast<int32_t>(steps.size());
+    return 0;
+}
+
+int cr_write_perfect_game(cr_solver* s, int32_t order, const char* path) {
+    if (int rc = check_round_table(s)) return rc;
+    if (path == nullptr) return fail(CR_ERR_ARG, "null path");
+    if (order != CR_GAME_ORDER_PY && order != CR_GAME_ORDER_CPP) return fail(CR_ERR_ARG, "game order %d", order);
+    HIP_TRY(hipSetDevice(s->device));
+    std::vector<cr::GameStep> steps;
+    if (int rc = crd::game_extract(s->gv, s->k, s->columns, s->d_table, s->g.matrix, s->cop_stay != 0,
+                                   s->robber_stay != 0, order == CR_GAME_ORDER_CPP, s->stream, steps))
+        return rc;
+    if (int rc = cr::write_game_json(path, s->k, steps)) return fail(rc, "could not write %s", path);
+    return 0;
+}
+
+int cr_cop_number(cr_solver* s, int32_t k_max, int32_t* cop_number, cr_verdict* verdict) {
+    if (int rc = check_solver(s)) return rc;
+    if (s->stage < kLoaded) return fail(CR_ERR_ORDER, "cop number before loadGraphFile");
+    if (k_max < 1 || k_max > CR_MAX_K) return fail(CR_ERR_K_RANGE, "k_max=%d outside 1..%d", k_max, CR_MAX_K);
+    HIP_TRY(hipSetDevice(s->device));
+    const std::vector<uint8_t> matrix = s->g.matrix;
+    const int32_t n = s->g.n, p = s->p;
+    if (cop_number) *cop_number = 0;
+    for (int32_t k = 1; k <= k_max; ++k) {
+        int rc = do_load(s, n, matrix.data(), k, p);
+        if (!rc) rc = do_build(s);
+        if (!rc) rc = do_init(s, nullptr);
+        if (!rc) rc = do_loop(s);
+        cr_verdict v{};
+        if (!rc) rc = cr_find_final_result(s, &v);
+        if (rc) return rc;
+        if (verdict) *verdict = v;
+        if (v.win) {
+            if (cop_number) *cop_number = k;
+            return 0;
+        }
     }
     return 0;
 }

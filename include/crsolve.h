@@ -110,6 +110,19 @@ int cr_load_graph_file(cr_solver* s, const char* filename, int32_t k, int32_t p)
 int cr_load_graph_matrix(cr_solver* s, int32_t n, const uint8_t* adj, int32_t k, int32_t p);
 /* Selects the solver TU being replaced (default CR_VARIANT_COLUMNS). */
 int cr_set_variant(cr_solver* s, int32_t variant);
+/* Movement rules, AuxGraph::setSelfEdges (AuxGraph.h:81-91): cops may stay on
+ * their vertex (an extra option per cop, AuxGraph.h:203-205) and/or the robber
+ * may stay (k_cops_visibility.cpp:253-255). The reference solvers fix both to
+ * FALSE (k_cops_visibility.cpp:63-64, the default here); the legacy round
+ * solver lets both stay (legacy/src/k_cops_rounds.cpp:112-118, robber :224-233).
+ * Takes effect at the next cr_build_aux_graph; the column variant only. */
+int cr_set_rules(cr_solver* s, int32_t cop_stay, int32_t robber_stay);
+/* 1-based edge-list input, legacy/util/edge_parser.py:4-63: one edge "u v"
+ * (or "u, v") per line, '#' comments, undirected, N = the largest id. Returns N
+ * (0 when there is no edge) and fills adj[N*N] when N <= cap. */
+int32_t cr_parse_edge_list(const uint8_t* text, uint64_t len, uint8_t* adj, int32_t cap);
+/* loadGraphFile from an edge list (the matrix edge_parser.py would write). */
+int cr_load_edge_file(cr_solver* s, const char* filename, int32_t k, int32_t p);
 
 /* ---- the six solver steps -------------------------------------------- */
 /* buildAuxGraph(), k_cops_visibility.cpp:59-83 -> AuxGraph::constructFrom
@@ -165,6 +178,41 @@ typedef struct {
 } cr_macro_stats;
 int cr_get_macro_stats(cr_solver* s, cr_macro_stats* out, uint64_t* nodes_per_step, uint64_t* edges_per_step,
                        int32_t cap);
+/* DP-table export (the assets/dp_tables/<graph>_<k>cops_dp_table.json layout of
+ * the legacy round solver, legacy/src/k_cops_rounds.cpp:211-281 and
+ * legacy/python/k_cops.py:86-131): a JSON list, Python json.dumps spacing, one
+ * object per (cId, r) in cId order: {"cops": [...], "robber": r, "cop_wins":
+ * true|false, "steps_to_win": s}, s = (depth + 1) / 2 for a won cop-turn state
+ * (a round is two plies), -1 otherwise. Solved column variant with p = 1. */
+int cr_write_dp_table(cr_solver* s, const char* path);
+/* Perfect game (assets/cached_solutions/<graph>_<k>cops_perfect_game.json).
+ * From the universal configuration with the least worst-case rounds (lowest
+ * cId on ties) and the robber start with the most rounds (lowest r on ties),
+ * the cops pick the move minimising the robber's best reply and the robber
+ * the reply maximising the rounds left, first best in the order of the
+ * reference producer chosen by `order`:
+ *   CR_GAME_ORDER_PY  legacy/python/k_cops.py:174-241: cop moves in
+ *                     itertools.product order over [neighbours ascending,
+ *                     stay], the robber's options likewise;
+ *   CR_GAME_ORDER_CPP legacy/src/k_cops_rounds.cpp:307-397: cop moves in
+ *                     ascending order of the next configuration, the robber's
+ *                     options its neighbours only.
+ * Walked on the device. turn: 0 = "Cop's Turn", 1 = "Robber's Turn",
+ * 2 = "Game Over - Captured!". length = 0 when no configuration wins. */
+enum cr_game_order { CR_GAME_ORDER_PY = 0, CR_GAME_ORDER_CPP = 1 };
+typedef struct {
+    uint8_t cops[CR_MAX_K];
+    int32_t robber;
+    int32_t turn;
+} cr_game_step;
+int cr_extract_game(cr_solver* s, int32_t order, cr_game_step* out, int32_t cap, int32_t* length);
+/* The same game as JSON in the reference's layout (json.dump, indent 4). */
+int cr_write_perfect_game(cr_solver* s, int32_t order, const char* path);
+/* Cop number: the least k in 1..k_max whose solve (same graph, variant, p and
+ * rules) is a WIN (the harness sweeps k for this, util/test_k_cops_visibility.py:
+ * 121-145). cop_number = 0 when none is; verdict is that solve's. The solver is
+ * left solved at the last k tried. */
+int cr_cop_number(cr_solver* s, int32_t k_max, int32_t* cop_number, cr_verdict* verdict);
 /* Total team moves sum over C of |T(C)|: the "Total edge pointers" the
  * reference prints after AuxGraph::createTransitions (AuxGraph.h:274). */
 int cr_count_transitions(cr_solver* s, uint64_t* total);

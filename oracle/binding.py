@@ -189,3 +189,105 @@ def pass_counts_from_depths(n, k, states, max_passes=8192):
     pc = np.zeros(max_passes, np.uint64)
     np_ = lib().or_pass_counts_from_depths(n, k, _p8(states), _p64(pc), max_passes)
     return [int(x) for x in pc[:np_]]
+
+
+# ---------------------------------------------------------------------------
+# Round-solver outputs (legacy/src/k_cops_rounds.cpp, legacy/python/k_cops.py).
+# Pure-Python restatements for small cases: the solve itself is v1_solve with
+# cop_stay = robber_stay = 1 (k_cops.py:33-36 gives every vertex a self-loop;
+# its synchronous passes, :86-131, are the level-synchronous depths), and a
+# cop-turn state won at depth t has (t + 1) // 2 rounds left (k_cops.py:124).
+# ---------------------------------------------------------------------------
+def dp_table_rows(n, k, states, columns=2):
+    """The dp_tables JSON rows: one dict per (cId, r) in cId order."""
+    cfg = configs(n, k)
+    st = np.asarray(states).reshape(len(cfg), n, columns)[:, :, 0].astype(np.int64)
+    rows = []
+    for c in range(len(cfg)):
+        cops = [int(x) for x in cfg[c]]
+        for r in range(n):
+            b = int(st[c, r])
+            win = bool(b & 1)
+            rows.append({"cops": cops, "robber": r, "cop_wins": win,
+                         "steps_to_win": ((b >> 1) + 1) // 2 if win else -1})
+    return rows
+
+
+def perfect_game(adj, k, states, cop_stay=1, robber_stay=1, columns=2, order="py"):
+    """Perfect-game extraction over a solved table. The start is the universal
+    configuration with the least worst case, lowest cId on ties, and the robber
+    start with the most rounds (first in r order).
+    order="py": legacy/python/k_cops.py:174-241 -- cop moves in
+      itertools.product order over [neighbours ascending, stay], the robber's
+      replies and moves over the same options (that producer iterates a set for
+      the start, :142, so its own tie order there is arbitrary);
+    order="cpp": legacy/src/k_cops_rounds.cpp:307-397 -- cop moves in ascending
+      order of the next configuration (its CSR transition list), the robber's
+      replies and moves over its neighbours only (no stay), no loop limit."""
+    adj = np.asarray(adj)
+    n = adj.shape[0]
+    cfg = [tuple(int(x) for x in c) for c in configs(n, k)]
+    index = {c: i for i, c in enumerate(cfg)}
+    st = np.asarray(states).reshape(len(cfg), n, columns)[:, :, 0]
+
+    def options(u, stay):
+        o = [v for v in range(n) if adj[u, v]]
+        if stay and not adj[u, u]:
+            o.append(u)
+        return o
+
+    cop_opt = [options(u, cop_stay) for u in range(n)]
+    rob_opt = [options(u, robber_stay and order == "py") for u in range(n)]
+    won = lambda c, r: bool(st[index[c], r] & 1)
+    rounds = lambda c, r: ((int(st[index[c], r]) >> 1) + 1) // 2
+    best, best_worst = None, None
+    for c in cfg:
+        if all(won(c, r) for r in range(n)):
+            w = max(rounds(c, r) for r in range(n))
+            if best_worst is None or w < best_worst:
+                best, best_worst = c, w
+    if best is None:
+        return []
+    r0, most = None, -1
+    for r in range(n):
+        if won(best, r) and rounds(best, r) > most:
+            most, r0 = rounds(best, r), r
+    path, cur, rob = [], best, r0
+    import itertools
+    loops = 0
+    while True:
+        if rob in cur:  # the robber walked onto a cop: the C++ producer records it (:325-327)
+            if order == "cpp":
+                path.append({"cops": list(cur), "robber": rob, "turn": "Game Over - Captured!"})
+            break
+        loops += 1
+        if order == "py" and loops > 5 * n:
+            break
+        path.append({"cops": list(cur), "robber": rob, "turn": "Cop's Turn"})
+        nxt, least = cur, None
+        moves = [tuple(sorted(m)) for m in itertools.product(*[cop_opt[c] for c in cur])]
+        if order == "cpp":
+            moves = sorted(set(moves), key=lambda c: index[c])
+        for nc in moves:
+            if rob in nc:
+                score, ok = 0, True
+            else:
+                score, ok = -1, True
+                for v in rob_opt[rob]:
+                    if not won(nc, v):
+                        ok = False
+                        break
+                    score = max(score, rounds(nc, v))
+            if ok and (least is None or score < least):
+                least, nxt = score, nc
+        cur = nxt
+        if rob in cur:
+            path.append({"cops": list(cur), "robber": rob, "turn": "Game Over - Captured!"})
+            break
+        path.append({"cops": list(cur), "robber": rob, "turn": "Robber's Turn"})
+        go, most = rob, -1
+        for v in rob_opt[rob]:
+            if won(cur, v) and rounds(cur, v) > most:
+                most, go = rounds(cur, v), v
+        rob = go
+    return path

@@ -122,10 +122,31 @@ def record(index, exe, tag, graph, k, p, mdir):
           f"passes={len(facts['passes'])} blob={len(blob)}", flush=True)
 
 
+def copy_assets():
+    """Edge lists (assets/edges), the DP-table JSON (assets/dp_tables, stored
+    gzip-compressed: the bytes are compared after decompression) and the
+    recorded perfect games (assets/cached_solutions)."""
+    import gzip
+    for sub in ("edges", "cached_solutions"):
+        os.makedirs(os.path.join(HERE, sub), exist_ok=True)
+        for fn in sorted(os.listdir(os.path.join(REF, "assets", sub))):
+            shutil.copyfile(os.path.join(REF, "assets", sub, fn), os.path.join(HERE, sub, fn))
+    os.makedirs(os.path.join(HERE, "dp_tables"), exist_ok=True)
+    for fn in sorted(os.listdir(os.path.join(REF, "assets", "dp_tables"))):
+        with open(os.path.join(REF, "assets", "dp_tables", fn), "rb") as f:
+            data = f.read()
+        with gzip.GzipFile(os.path.join(HERE, "dp_tables", fn + ".gz"), "wb", mtime=0) as g:
+            g.write(data)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--with-slow", action="store_true")
+    ap.add_argument("--assets-only", action="store_true", help="only copy the reference's data files")
     args = ap.parse_args()
+    if args.assets_only:
+        copy_assets()
+        return
     if not os.path.exists(os.path.join(BIN, "k_cops_visibility")):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
     mdir = os.path.join(HERE, "matrices")
@@ -136,6 +157,8 @@ def main():
         shutil.copyfile(os.path.join(REF, "assets", "matrices", fn), os.path.join(mdir, fn))
     for fn in ("test_results.csv", "test_results_v2.csv", "test_results_affected.csv"):
         shutil.copyfile(os.path.join(REF, "console_outputs", fn), os.path.join(HERE, "csv", fn))
+    # the reference's data files for the edge-list input and the round-solver outputs
+    copy_assets()
     # generated workloads (written by the package's own seeded generators)
     sys.path.insert(0, REPO)
     from copsandrobbers_amd import workloads

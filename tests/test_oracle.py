@@ -154,6 +154,51 @@ def test_levelsync_equals_pass_sweep(oracle, name):
     assert np.array_equal(hist, levels.astype(np.int64))
 
 
+def passes_from_column_depths(states, n, k, cols, C):
+    """Per-pass counts of the 2p-column sweep from the depth table alone: a
+    state of column c and depth t > 0 is marked in pass
+    t - (c + t) // cols + [(c + t) % cols == 0] (the device column solver's
+    rule). Also returns the (column, depth) pairs whose successor chain would
+    not end on a capture column (0 or 1); the rule needs there to be none."""
+    st = states.reshape(C, n, cols)
+    per_pass, odd = {}, []
+    for c in range(cols):
+        col = st[:, :, c]
+        d = (col >> 1)[(col & 1) == 1]
+        for t, cnt in enumerate(np.bincount(d, minlength=128)):
+            if t == 0 or cnt == 0:
+                continue
+            if (c + t) % cols not in (0, 1):
+                odd.append((c, t))
+            p = t - (c + t) // cols + (1 if (c + t) % cols == 0 else 0)
+            per_pass[p] = per_pass.get(p, 0) + int(cnt)
+    last = max(per_pass, default=0)
+    return [per_pass.get(i, 0) for i in range(1, last + 1)] + [0], odd
+
+
+@pytest.mark.parametrize("name", sorted(n for n, v in INDEX.items() if v["solver"] == "v1" and v["p"] >= 2))
+def test_column_pass_rule_reproduces_reference_passes(oracle, name):
+    """The pass counts the device column solver derives from its depth
+    histogram equal the reference's printed passes on every golden p >= 2 case."""
+    e = INDEX[name]
+    n, adj = load(oracle, e["graph"])
+    s = oracle.v1_solve(adj, e["k"], e["p"])
+    passes, odd = passes_from_column_depths(s.states, n, e["k"], 2 * e["p"], oracle.config_count(n, e["k"]))
+    assert odd == []
+    assert passes == e["passes"] == s.pass_counts
+
+
+def test_column_pass_rule_detects_a_planted_depth(oracle):
+    n, adj = load(oracle, "grid9")
+    s = oracle.v1_solve(adj, 2, 2)
+    bad = s.states.copy()
+    i = int(np.flatnonzero((bad & 1) & ((bad >> 1) > 3))[0])
+    bad[i] = np.uint8(bad[i] + 2)  # one state one level deeper
+    C = oracle.config_count(n, 2)
+    passes, odd = passes_from_column_depths(bad, n, 2, 4, C)
+    assert odd or passes != INDEX["v1__grid9__k2__p2"]["passes"]
+
+
 @pytest.mark.parametrize("name", [n for n, v in INDEX.items() if v["solver"] == "v1" and not v["arrays"]])
 def test_large_v1_tables_match_reference_hash(oracle, name):
     e = INDEX[name]
