@@ -354,7 +354,43 @@ struct SepStage {
     int32_t stage;        // i, 1..K
 };
 
-template <int K>
+// Loads / stores of one W-word row: 16-byte vector accesses where the row is
+// 16-byte aligned (W = 2, 4).
+template <int W>
+__device__ __forceinline__ void load_row(const uint64_t* __restrict__ src, uint64_t* v) {
+    if constexpr (W == 4) {
+        const ulonglong2 x = reinterpret_cast<const ulonglong2*>(src)[0];
+        const ulonglong2 y = reinterpret_cast<const ulonglong2*>(src)[1];
+        v[0] = x.x;
+        v[1] = x.y;
+        v[2] = y.x;
+        v[3] = y.y;
+    } else if constexpr (W == 2) {
+        const ulonglong2 x = reinterpret_cast<const ulonglong2*>(src)[0];
+        v[0] = x.x;
+        v[1] = x.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < W; ++q) v[q] = src[q];
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void store_row(uint64_t* __restrict__ dst, const uint64_t* v) {
+    if constexpr (W == 4) {
+        reinterpret_cast<ulonglong2*>(dst)[0] = make_ulonglong2(v[0], v[1]);
+        reinterpret_cast<ulonglong2*>(dst)[1] = make_ulonglong2(v[2], v[3]);
+    } else if constexpr (W == 2) {
+        reinterpret_cast<ulonglong2*>(dst)[0] = make_ulonglong2(v[0], v[1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < W; ++q) dst[q] = v[q];
+    }
+}
+
+constexpr int kPullMlp = 8;  // input rows whose loads a thread keeps in flight
+
+template <int K, int W>
 __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a, SepStage st) {
     extern __shared__ __align__(16) uint8_t lds[];
     if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
@@ -362,9 +398,10 @@ __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a,
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const uint16_t* out_off = reinterpret_cast<const uint16_t*>(lds + gv.off_out_off);
     const uint8_t* out_nbr = lds + gv.off_out_nbr;
-    const int W = gv.W, n = gv.n, i = st.stage, jb = K - i;
-    uint64_t full[4];
-    for (int q = 0; q < 4; ++q) {
+    const int n = gv.n, i = st.stage, jb = K - i;
+    uint64_t full[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
         const int bits = min(64, max(0, n - 64 * q));
         full[q] = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
     }
@@ -376,29 +413,57 @@ __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a,
         uint32_t A[K], B[K + 1];
         unrank_j(ra, i, n, binom, gv.M, A);
         unrank_j(rb, jb, n, binom, gv.M, B);
-        const uint32_t ra_in = rank_j(A, i - 1, n, binom, gv.M);  // A without its largest element
-        const uint32_t c = A[i - 1];
-        uint64_t acc[4] = {0, 0, 0, 0};
+        const uint64_t row_in = static_cast<uint64_t>(rank_j(A, i - 1, n, binom, gv.M)) * st.cnt_b_in;
+        const uint32_t c = A[i - 1];  // the moved cop: A's largest element
+        uint64_t acc[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) acc[q] = 0;
         const int beg = out_off[c], end = out_off[c + 1];
-        for (int e = beg; e < end; ++e) {
-            const uint32_t v = out_nbr[e];
-            // B + {v}, sorted
-            uint32_t M2[K + 1];
-            int pos = 0;
-            bool placed = false;
-            for (int t = 0; t < jb; ++t) {
-                if (!placed && v <= B[t]) { M2[pos++] = v; placed = true; }
-                M2[pos++] = B[t];
+        for (int e0 = beg; e0 < end; e0 += kPullMlp) {
+            // ranks of up to kPullMlp input rows first, then their loads together
+            uint32_t idx[kPullMlp];
+            const int m = min(kPullMlp, end - e0);
+#pragma unroll
+            for (int u = 0; u < kPullMlp; ++u) {
+                idx[u] = 0;
+                if (u >= m) continue;
+                const uint32_t v = out_nbr[e0 + u];
+                // B + {v}, sorted
+                uint32_t M2[K + 1];
+                int pos = 0;
+                bool placed = false;
+                for (int t = 0; t < jb; ++t) {
+                    if (!placed && v <= B[t]) {
+                        M2[pos++] = v;
+                        placed = true;
+                    }
+                    M2[pos++] = B[t];
+                }
+                if (!placed) M2[pos++] = v;
+                idx[u] = rank_j(M2, jb + 1, n, binom, gv.M);
             }
-            if (!placed) M2[pos++] = v;
-            const uint64_t r_in = static_cast<uint64_t>(ra_in) * st.cnt_b_in + rank_j(M2, jb + 1, n, binom, gv.M);
-            for (int q = 0; q < W; ++q) acc[q] |= st.in[r_in * W + q];
+            uint64_t got[kPullMlp][W];
+#pragma unroll
+            for (int u = 0; u < kPullMlp; ++u) {
+                if (u < m) {
+                    load_row<W>(st.in + (row_in + idx[u]) * W, got[u]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < W; ++q) got[u][q] = 0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kPullMlp; ++u)
+#pragma unroll
+                for (int q = 0; q < W; ++q) acc[q] |= got[u][q];
         }
-        if (i < K) {
-            for (int q = 0; q < W; ++q) st.out[o * W + q] = acc[q];
-        } else {  // candidates of configuration ra: only states not won yet
-            for (int q = 0; q < W; ++q) st.out[o * W + q] = acc[q] & ~a.won0[o * W + q] & full[q];
+        if (i == K) {  // candidates of configuration ra: only states not won yet
+            uint64_t w0[W];
+            load_row<W>(a.won0 + o * W, w0);
+#pragma unroll
+            for (int q = 0; q < W; ++q) acc[q] &= ~w0[q] & full[q];
         }
+        store_row<W>(st.out + o * W, acc);
     }
 }
 
@@ -1145,30 +1210,41 @@ struct LaunchPush {
     }
 };
 
+template <int K, int W>
+int launch_pull(cr_solver* s, LevelArgs a) {
+    if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_sep_stage<K, W>), s->gv.image_bytes)) return rc;
+    const uint64_t* in = s->d_fr1;
+    for (int i = 1; i <= K; ++i) {
+        SepStage st{};
+        st.stage = i;
+        st.in = in;
+        st.out = i == K ? s->d_nx0 : s->d_sep[(i - 1) & 1];
+        st.cnt_b = static_cast<uint32_t>(s->ms_count[K - i]);
+        st.cnt_b_in = static_cast<uint32_t>(s->ms_count[K - i + 1]);
+        st.rows = s->ms_count[i] * s->ms_count[K - i];
+        const uint64_t blocks = std::min<uint64_t>((st.rows + kBlock - 1) / kBlock, 16384);
+        Timer* t = start_timer(s, "sep_stage");
+        hipLaunchKernelGGL((k_sep_stage<K, W>), dim3(std::max<uint64_t>(blocks, 1)), dim3(kBlock), s->gv.image_bytes,
+                           s->stream, s->gv, a, st);
+        t->stop();
+        in = st.out;
+    }
+    Timer* t = start_timer(s, "clear_frontier");
+    hipLaunchKernelGGL(k_clear_frontier, dim3(1024), dim3(kBlock), 0, s->stream, a, s->W);
+    t->stop();
+    return 0;
+}
+
 template <int K>
 struct LaunchPull {
     static int run(cr_solver* s, LevelArgs a) {
-        if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_sep_stage<K>), s->gv.image_bytes)) return rc;
-        const uint64_t* in = s->d_fr1;
-        for (int i = 1; i <= K; ++i) {
-            SepStage st{};
-            st.stage = i;
-            st.in = in;
-            st.out = i == K ? s->d_nx0 : s->d_sep[(i - 1) & 1];
-            st.cnt_b = static_cast<uint32_t>(s->ms_count[K - i]);
-            st.cnt_b_in = static_cast<uint32_t>(s->ms_count[K - i + 1]);
-            st.rows = s->ms_count[i] * s->ms_count[K - i];
-            const uint64_t blocks = std::min<uint64_t>((st.rows + kBlock - 1) / kBlock, 16384);
-            Timer* t = start_timer(s, "sep_stage");
-            hipLaunchKernelGGL(k_sep_stage<K>, dim3(std::max<uint64_t>(blocks, 1)), dim3(kBlock), s->gv.image_bytes,
-                               s->stream, s->gv, a, st);
-            t->stop();
-            in = st.out;
+        switch (s->W) {
+            case 1: return launch_pull<K, 1>(s, a);
+            case 2: return launch_pull<K, 2>(s, a);
+            case 3: return launch_pull<K, 3>(s, a);
+            case 4: return launch_pull<K, 4>(s, a);
+            default: return fail(CR_ERR_ARG, "row width %d", s->W);
         }
-        Timer* t = start_timer(s, "clear_frontier");
-        hipLaunchKernelGGL(k_clear_frontier, dim3(1024), dim3(kBlock), 0, s->stream, a, s->W);
-        t->stop();
-        return 0;
     }
 };
 
