@@ -12,6 +12,7 @@ Usage: python -m copsandrobbers_amd.build
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
@@ -40,13 +41,26 @@ def run(cmd):
         raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}")
 
 
+HIP_SOURCES = ("crsolve.hip", "macro.hip", "batch.hip", "columns.hip", "game.hip", "cr_host.cpp")
+
+
 def build_library():
+    """Compiles every source to an object (in parallel; each holds the kernels it
+    launches) and links them into libcrsolve.so."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = max(1, min(len(HIP_SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+
+    def compile_one(name):
+        obj = os.path.join(objdir, os.path.splitext(name)[0] + ".o")
+        run([HIPCC, *HIP_FLAGS, "-c", f"-I{INCLUDE}", f"-I{CSRC}", os.path.join(CSRC, name), "-o", obj])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        objs = list(pool.map(compile_one, HIP_SOURCES))
     out = os.path.join(HERE, "libcrsolve.so")
     tmp = out + ".tmp"
-    run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}",
-         os.path.join(CSRC, "crsolve.hip"), os.path.join(CSRC, "macro.hip"), os.path.join(CSRC, "batch.hip"),
-         os.path.join(CSRC, "columns.hip"), os.path.join(CSRC, "game.hip"), os.path.join(CSRC, "cr_host.cpp"),
-         "-o", tmp])
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -350,8 +350,11 @@ struct SepStage {
     uint64_t* out;        // H_i, rows [cnt(i) * cnt(K-i)] (stage K: nx0)
     uint32_t cnt_b;       // cnt(K-i): B-multisets of H_i
     uint32_t cnt_b_in;    // cnt(K-i+1): B-multisets of H_{i-1}
+    uint32_t cnt_a;       // cnt(i): A-multisets of H_i
+    uint32_t cnt_a_in;    // cnt(i-1): A-multisets of H_{i-1}
     uint64_t rows;        // rows of H_i
     int32_t stage;        // i, 1..K
+    uint32_t r;           // k_pull_tile: rows of each input block per tile
 };
 
 // Loads / stores of one W-word row: 16-byte vector accesses where the row is
@@ -390,7 +393,33 @@ __device__ __forceinline__ void store_row(uint64_t* __restrict__ dst, const uint
 
 constexpr int kPullMlp = 8;  // input rows whose loads a thread keeps in flight
 
-template <int K, int W>
+// Lexicographic successor of a non-decreasing J-tuple over [0, n), in place
+// (the order rank_j counts in); false after the last tuple.
+template <int J>
+__device__ __forceinline__ bool ms_next(uint32_t* t, uint32_t n) {
+#pragma unroll
+    for (int i = J - 1; i >= 0; --i) {
+        if (t[i] + 1 < n) {
+            const uint32_t v = t[i] + 1;
+#pragma unroll
+            for (int j = i; j < J; ++j) t[j] = v;
+            return true;
+        }
+    }
+    return false;
+}
+
+// Stage I of the pull (J = K - I cops not moved yet), one row of H_I per
+// lane. The intermediate tables H_1..H_{K-1} are laid out with B major: row
+// rank(B) * cnt(I) + rank(A) (H_0 and H_K are indexed by configuration). Rows
+// are taken 64 at a time; consecutive rows differ only in A's last element as
+// long as it does not wrap, so a wave unranks the first row of its window once
+// and walks these runs, handing each lane its (A, B) pair. A neighbour's
+// input row then costs J compares and one binomial lookup: with T(p, x) the
+// rank term of value x at position p of a (J+1)-multiset,
+//   rank(B + v) = rank0 - sum_{B_i < v} T(i, B_i) - T(q, v) - sum_{B_i >= v} T(i + 1, B_i)
+// where q counts the B_i below v (rank_j's formula with v merged in).
+template <int I, int J, int W>
 __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a, SepStage st) {
     extern __shared__ __align__(16) uint8_t lds[];
     if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
@@ -398,29 +427,71 @@ __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a,
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const uint16_t* out_off = reinterpret_cast<const uint16_t*>(lds + gv.off_out_off);
     const uint8_t* out_nbr = lds + gv.off_out_nbr;
-    const int n = gv.n, i = st.stage, jb = K - i;
+    const uint32_t n = gv.n;
+    const uint32_t Mp = gv.M + 1;  // row stride of the binomial table
+    const int lane = threadIdx.x & 63;
     uint64_t full[W];
 #pragma unroll
     for (int q = 0; q < W; ++q) {
-        const int bits = min(64, max(0, n - 64 * q));
+        const int bits = min(64, max(0, static_cast<int>(n) - 64 * q));
         full[q] = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
     }
-    for (uint64_t o = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; o < st.rows;
-         o += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        // rows < 2^32 here (alloc_pull caps the intermediates): 32-bit division
-        const uint32_t o32 = static_cast<uint32_t>(o);
-        const uint32_t ra = o32 / st.cnt_b, rb = o32 - ra * st.cnt_b;
-        uint32_t A[K], B[K + 1];
-        unrank_j(ra, i, n, binom, gv.M, A);
-        unrank_j(rb, jb, n, binom, gv.M, B);
-        const uint64_t row_in = static_cast<uint64_t>(rank_j(A, i - 1, n, binom, gv.M)) * st.cnt_b_in;
-        const uint32_t c = A[i - 1];  // the moved cop: A's largest element
+    const uint32_t nwin = static_cast<uint32_t>((st.rows + 63) / 64);
+    const XcdSplit xs = xcd_split(nwin);
+    for (uint32_t win = xs.first; win < xs.end; win += xs.stride) {
+        const uint32_t o0 = win * 64;
+        uint32_t A[I], B[J > 0 ? J : 1];
+        {  // runs over A's last element; A wraps into the next B
+            uint32_t ca[I], cb[J > 0 ? J : 1];
+            uint32_t ra0 = o0;
+            if constexpr (J > 0) {
+                const uint32_t rb0 = o0 / st.cnt_a;
+                ra0 = o0 - rb0 * st.cnt_a;
+                unrank_j(rb0, J, n, binom, gv.M, cb);
+            }
+            unrank_j(ra0, I, n, binom, gv.M, ca);
+            for (int s = 0;;) {
+                const uint32_t x0 = ca[I - 1];
+                const int len = static_cast<int>(n - x0);
+                if (lane >= s && lane < s + len) {
+#pragma unroll
+                    for (int t = 0; t < I; ++t) A[t] = ca[t];
+                    A[I - 1] = x0 + static_cast<uint32_t>(lane - s);
+#pragma unroll
+                    for (int t = 0; t < J; ++t) B[t] = cb[t];
+                }
+                s += len;
+                if (s >= 64) break;
+                ca[I - 1] = n - 1;
+                if (!ms_next<I>(ca, n)) {
+#pragma unroll
+                    for (int t = 0; t < I; ++t) ca[t] = 0;
+                    if constexpr (J > 0) ms_next<J>(cb, n);
+                }
+            }
+        }
+        const uint64_t o = static_cast<uint64_t>(o0) + lane;
+        if (o >= st.rows) continue;
+        const uint32_t c = A[I - 1];  // the cop moved by this stage: A's largest element
+        // input row of (A without c, B + v): row_in + rank(B + v) * in_stride
+        uint64_t row_in = 0;
+        const uint64_t in_stride = st.cnt_a_in;
+        if constexpr (I > 1) row_in = rank_j(A, I - 1, n, binom, gv.M);
+        // rank terms of B's elements at their own and at the next position
+        constexpr int J1 = J + 1;
+        const uint32_t M1 = n + J;  // M of (J+1)-multisets
+        uint32_t U[J > 0 ? J : 1], S[J > 0 ? J : 1];
+#pragma unroll
+        for (int t = 0; t < J; ++t) {
+            U[t] = binom[(J1 - t) * Mp + (M1 - 1 - B[t] - t)];
+            S[t] = binom[(J - t) * Mp + (M1 - 2 - B[t] - t)];
+        }
+        const uint32_t rank0 = binom[J1 * Mp + M1] - 1;
         uint64_t acc[W];
 #pragma unroll
         for (int q = 0; q < W; ++q) acc[q] = 0;
         const int beg = out_off[c], end = out_off[c + 1];
         for (int e0 = beg; e0 < end; e0 += kPullMlp) {
-            // ranks of up to kPullMlp input rows first, then their loads together
             uint32_t idx[kPullMlp];
             const int m = min(kPullMlp, end - e0);
 #pragma unroll
@@ -428,25 +499,24 @@ __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a,
                 idx[u] = 0;
                 if (u >= m) continue;
                 const uint32_t v = out_nbr[e0 + u];
-                // B + {v}, sorted
-                uint32_t M2[K + 1];
-                int pos = 0;
-                bool placed = false;
-                for (int t = 0; t < jb; ++t) {
-                    if (!placed && v <= B[t]) {
-                        M2[pos++] = v;
-                        placed = true;
+                if constexpr (J == 0) {
+                    idx[u] = v;
+                } else {
+                    uint32_t sum = 0, q = 0;
+#pragma unroll
+                    for (int t = 0; t < J; ++t) {
+                        const bool lt = B[t] < v;
+                        sum += lt ? U[t] : S[t];
+                        q += lt ? 1u : 0u;
                     }
-                    M2[pos++] = B[t];
+                    idx[u] = rank0 - sum - binom[(J1 - q) * Mp + (M1 - 1 - v - q)];
                 }
-                if (!placed) M2[pos++] = v;
-                idx[u] = rank_j(M2, jb + 1, n, binom, gv.M);
             }
             uint64_t got[kPullMlp][W];
 #pragma unroll
             for (int u = 0; u < kPullMlp; ++u) {
                 if (u < m) {
-                    load_row<W>(st.in + (row_in + idx[u]) * W, got[u]);
+                    load_row<W>(st.in + (row_in + idx[u] * in_stride) * W, got[u]);
                 } else {
 #pragma unroll
                     for (int q = 0; q < W; ++q) got[u][q] = 0;
@@ -457,13 +527,202 @@ __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a,
 #pragma unroll
                 for (int q = 0; q < W; ++q) acc[q] |= got[u][q];
         }
-        if (i == K) {  // candidates of configuration ra: only states not won yet
+        if constexpr (J == 0) {  // candidates of configuration o: only states not won yet
             uint64_t w0[W];
             load_row<W>(a.won0 + o * W, w0);
 #pragma unroll
             for (int q = 0; q < W; ++q) acc[q] &= ~w0[q] & full[q];
         }
         store_row<W>(st.out + o * W, acc);
+    }
+}
+
+// Stage I of the pull by LDS tiles. With B major, the input rows of a fixed B
+// are n contiguous blocks of H_{I-1} (block B + v holds the rows of every A'),
+// so a tile (B, A'-range [ra0, ra0 + r)) loads rows ra0.. of all n blocks
+// into LDS with coalesced reads, then every output row (A' + c, B) of the tile,
+// c >= max A', ORs its neighbours' rows out of LDS:
+//   H_I[A' + c][B] = OR_{v in N(c)} tile[v][A'].
+// For fixed A' the outputs of increasing c are consecutive rows and the
+// output runs of consecutive A' are adjacent, so consecutive tiles write
+// consecutive ranges of H_I. Each input row is loaded once per B that it
+// extends (at most J + 1 times).
+// A workgroup walks a contiguous chunk of tiles: B, A' and the output row
+// advance incrementally (one unrank per chunk), and the loads of the next
+// tile are held in registers while the current tile is computed from LDS
+// (two tile buffers after the graph image, n * r * W words each).
+constexpr int kTileMaxRows = 16;
+constexpr int kTileLoads = 8;  // input words a thread stages per tile: n * r * W <= kTileLoads * kBlock
+constexpr uint32_t kLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
+
+template <int I, int J, int W>
+__global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a, SepStage st) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint32_t t_lo[2][kTileMaxRows], t_pre[2][kTileMaxRows + 1];
+    if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
+    load_image(lds, gv.image, gv.image_bytes);
+    const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
+    const uint16_t* out_off = reinterpret_cast<const uint16_t*>(lds + gv.off_out_off);
+    const uint8_t* out_nbr = lds + gv.off_out_nbr;
+    const uint32_t n = gv.n;
+    const uint32_t Mp = gv.M + 1;
+    const uint32_t r = st.r, rW = r * W, words = n * rW;
+    uint64_t* buf[2];
+    buf[0] = reinterpret_cast<uint64_t*>(lds + ((gv.image_bytes + 15) & ~15u));
+    buf[1] = buf[0] + words;
+    uint64_t full[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        const int bits = min(64, max(0, static_cast<int>(n) - 64 * q));
+        full[q] = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+    }
+    const uint32_t cnt_in = st.cnt_a_in;        // rows per block: cnt(I - 1)
+    const uint32_t ntr = (cnt_in + r - 1) / r;  // A'-ranges per B
+    const uint64_t ntiles = static_cast<uint64_t>(st.cnt_b) * ntr;
+    // tiles in 8 contiguous segments, one per XCD (the grid is a multiple of
+    // 8), each split into contiguous chunks, one per workgroup
+    const uint32_t x = blockIdx.x % kXcds, lb = blockIdx.x / kXcds, bpx = gridDim.x / kXcds;
+    const uint64_t s0 = ntiles * x / kXcds, s1 = ntiles * (x + 1) / kXcds;
+    const uint64_t my0 = s0 + (s1 - s0) * lb / bpx, my1 = s0 + (s1 - s0) * (lb + 1) / bpx;
+    if (my0 >= my1) return;  // uniform per workgroup
+    const uint32_t count = static_cast<uint32_t>(my1 - my0);
+
+    // walk state (uniform): B with its rank terms, the tile's first A' row,
+    // and (thread 0) the A' themselves and the output row of the tile
+    constexpr int J1 = J + 1;
+    const uint32_t M1 = n + J;  // M of (J+1)-multisets
+    const uint32_t rank0 = binom[J1 * Mp + M1] - 1;
+    uint32_t rb = static_cast<uint32_t>(my0 / ntr);
+    uint32_t ra0 = static_cast<uint32_t>(my0 - static_cast<uint64_t>(rb) * ntr) * r;
+    uint32_t B[J > 0 ? J : 1], U[J > 0 ? J : 1], S[J > 0 ? J : 1];
+    if constexpr (J > 0) unrank_j(rb, J, n, binom, gv.M, B);
+    auto terms = [&]() {
+#pragma unroll
+        for (int t = 0; t < J; ++t) {
+            U[t] = binom[(J1 - t) * Mp + (M1 - 1 - B[t] - t)];
+            S[t] = binom[(J - t) * Mp + (M1 - 2 - B[t] - t)];
+        }
+    };
+    terms();
+    uint32_t Ap[I > 1 ? I - 1 : 1];
+    uint64_t out_row = 0;  // output row of the current tile's first output
+    if (threadIdx.x == 0) {
+        uint32_t Af[I];
+        if constexpr (I > 1) {
+            unrank_j(ra0, I - 1, n, binom, gv.M, Ap);
+#pragma unroll
+            for (int t = 0; t < I - 1; ++t) Af[t] = Ap[t];
+            Af[I - 1] = Ap[I - 2];
+        } else {
+            Af[0] = 0;
+        }
+        out_row = static_cast<uint64_t>(rb) * st.cnt_a + rank_j(Af, I, n, binom, gv.M);
+    }
+    // thread 0: the tile's rows t -> last element of A'_t and output offsets
+    auto tile_table = [&](int b, uint32_t rc) {
+        uint32_t pre = 0;
+        for (uint32_t t = 0; t < rc; ++t) {
+            const uint32_t lo = I > 1 ? Ap[I > 1 ? I - 2 : 0] : 0u;
+            t_lo[b][t] = lo;
+            t_pre[b][t] = pre;
+            pre += n - lo;
+            if constexpr (I > 1) ms_next<I - 1>(Ap, n);
+        }
+        t_pre[b][rc] = pre;
+    };
+    // the tile's input words (rows ra0.. of the n blocks) into registers
+    uint64_t stage_w[kTileLoads];
+    auto load_tile = [&](uint32_t rc) {
+#pragma unroll
+        for (int m = 0; m < kTileLoads; ++m) {
+            const uint32_t i = threadIdx.x + m * kBlock;
+            stage_w[m] = 0;
+            if (i < words) {
+                const uint32_t v = i / rW, w = i - v * rW;
+                if (w < rc * W) {
+                    uint32_t idx = v;
+                    if constexpr (J > 0) {
+                        uint32_t sum = 0, q = 0;
+#pragma unroll
+                        for (int t = 0; t < J; ++t) {
+                            const bool lt = B[t] < v;
+                            sum += lt ? U[t] : S[t];
+                            q += lt ? 1u : 0u;
+                        }
+                        idx = rank0 - sum - binom[(J1 - q) * Mp + (M1 - 1 - v - q)];
+                    }
+                    stage_w[m] = st.in[(static_cast<uint64_t>(idx) * cnt_in + ra0) * W + w];
+                }
+            }
+        }
+    };
+    auto store_tile = [&](int b) {
+#pragma unroll
+        for (int m = 0; m < kTileLoads; ++m) {
+            const uint32_t i = threadIdx.x + m * kBlock;
+            if (i < words) buf[b][i] = stage_w[m];
+        }
+    };
+
+    uint32_t rc = min(r, cnt_in - ra0);
+    if (threadIdx.x == 0) tile_table(0, rc);
+    load_tile(rc);
+    store_tile(0);
+    __shared__ unsigned long long t_out[2];
+    if (threadIdx.x == 0) t_out[0] = out_row;
+    for (uint32_t k = 0; k < count; ++k) {
+        const int cur = k & 1, nxt = cur ^ 1;
+        __syncthreads();  // tile k is in buf[cur]; tile k - 1 is finished with buf[nxt]
+        const uint32_t rc_cur = rc;
+        const bool more = k + 1 < count;
+        if (more) {  // advance to tile k + 1 and start its loads
+            ra0 += r;
+            if (ra0 >= cnt_in) {
+                ra0 = 0;
+                ++rb;
+                if constexpr (J > 0) {
+                    ms_next<J>(B, n);
+                    terms();
+                }
+                if constexpr (I > 1) {
+                    if (threadIdx.x == 0)
+#pragma unroll
+                        for (int t = 0; t < I - 1; ++t) Ap[t] = 0;
+                }
+            }
+            rc = min(r, cnt_in - ra0);
+            if (threadIdx.x == 0) {
+                tile_table(nxt, rc);
+                t_out[nxt] = t_out[cur] + t_pre[cur][rc_cur];
+            }
+            load_tile(rc);
+        }
+        const uint64_t* tb = buf[cur];
+        const uint32_t total = t_pre[cur][rc_cur];
+        const uint64_t o0 = t_out[cur];
+        for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) {
+            uint32_t t = 0;
+            while (t + 1 < rc_cur && t_pre[cur][t + 1] <= j) ++t;
+            const uint32_t c = t_lo[cur][t] + (j - t_pre[cur][t]);
+            uint64_t acc[W];
+#pragma unroll
+            for (int q = 0; q < W; ++q) acc[q] = 0;
+            const int beg = out_off[c], end = out_off[c + 1];
+            for (int e = beg; e < end; ++e) {
+                const uint64_t* src = tb + out_nbr[e] * rW + t * W;
+#pragma unroll
+                for (int q = 0; q < W; ++q) acc[q] |= src[q];
+            }
+            const uint64_t o = o0 + j;
+            if constexpr (J == 0) {  // candidates of configuration o: only states not won yet
+                uint64_t w0[W];
+                load_row<W>(a.won0 + o * W, w0);
+#pragma unroll
+                for (int q = 0; q < W; ++q) acc[q] &= ~w0[q] & full[q];
+            }
+            store_row<W>(st.out + o * W, acc);
+        }
+        if (more) store_tile(nxt);
     }
 }
 
@@ -799,17 +1058,22 @@ __global__ __launch_bounds__(kBlock) void k_cop_push_csr(GraphView gv, LevelArgs
 
 // ---------------------------------------------------------------------------
 // verdict (k_cops_visibility.cpp:301-354): universal rows -> min worst depth,
-// lowest cId on ties; packed (worst << 32 | cId) atomicMin
+// lowest cId on ties. Lanes = configurations; each lane scans its own row
+// (16-byte loads where the two-column rows allow), the wave keeps the least
+// packed (worst << 32 | cId) and issues one atomicMin.
 // ---------------------------------------------------------------------------
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_verdict(GraphView gv, LevelArgs a, int32_t columns) {
     const int lane = threadIdx.x & 63;
+    const int n = gv.n;
     uint64_t full[W];
 #pragma unroll
     for (int q = 0; q < W; ++q) {
-        const int bits = min(64, gv.n - 64 * q);
+        const int bits = min(64, n - 64 * q);
         full[q] = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
     }
+    const bool vec = columns == 2 && (n & 7) == 0;  // rows of 2n bytes, 16-byte aligned
+    unsigned long long best = ~0ull;
     const uint32_t nbatch = (gv.C + 63) / 64;
     for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
          b += gridDim.x * kWavesPerBlock) {
@@ -817,18 +1081,24 @@ __global__ __launch_bounds__(kBlock) void k_verdict(GraphView gv, LevelArgs a, i
         bool uni = row < gv.C;
 #pragma unroll
         for (int q = 0; q < W; ++q) uni = uni && a.won0[static_cast<uint64_t>(row) * W + q] == full[q];
-        uint64_t bal = __ballot(uni);
-        while (bal) {
-            const int src = __builtin_ctzll(bal);
-            bal &= bal - 1;
-            const uint32_t rr = b * 64 + src;
-            uint32_t worst = 0;
-            for (int r = lane; r < gv.n; r += 64)
-                worst = max(worst, static_cast<uint32_t>(a.table[(static_cast<uint64_t>(rr) * gv.n + r) * columns] >> 1));
-            for (int off = 32; off > 0; off >>= 1) worst = max(worst, static_cast<uint32_t>(__shfl_xor(worst, off)));
-            if (lane == 0) atomicMin(&a.ctl->verdict, (static_cast<unsigned long long>(worst) << 32) | rr);
+        if (!uni) continue;
+        const uint8_t* t = a.table + static_cast<uint64_t>(row) * n * columns;
+        uint32_t top = 0;  // largest cop-turn byte of the row
+        if (vec) {
+            const uint4* v = reinterpret_cast<const uint4*>(t);
+            for (int i = 0; i < n / 8; ++i) {
+                const uint4 x = v[i];
+                const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int h = 0; h < 4; ++h) top = max(top, max(w[h] & 0xFFu, (w[h] >> 16) & 0xFFu));
+            }
+        } else {
+            for (int r = 0; r < n; ++r) top = max(top, static_cast<uint32_t>(t[static_cast<uint64_t>(r) * columns]));
         }
+        best = min(best, (static_cast<unsigned long long>(top >> 1) << 32) | row);
     }
+    for (int off = 32; off > 0; off >>= 1) best = min(best, static_cast<unsigned long long>(__shfl_xor(best, off)));
+    if (lane == 0 && best != ~0ull) atomicMin(&a.ctl->verdict, best);
 }
 
 // ---------------------------------------------------------------------------
@@ -999,6 +1269,9 @@ struct cr_solver {
     uint64_t pull_scale = 500;    // pull threshold scale (permille of the estimated break-even)
     uint64_t* d_sep[2] = {nullptr, nullptr};  // factorised-pull intermediates H_i
     uint64_t ms_count[CR_MAX_K + 1] = {};     // C(n+j-1, j): multisets of size j
+    bool pull_tiles = true;                   // k_pull_tile (else k_sep_stage) runs the pull stages
+    uint32_t tile_rows = 2;                   // k_pull_tile: rows of each input block per tile
+    uint32_t tile_blocks = 8192;              // k_pull_tile: workgroups (each walks a chunk of tiles)
     uint32_t pull_rows = 0xffffffffu;         // frontier rows above which a level pulls
     uint32_t push_blocks = 2048;
     uint32_t row_blocks = 2048;
@@ -1210,25 +1483,55 @@ struct LaunchPush {
     }
 };
 
+// Stages I..K of one pull; stage I reads `in` (H_{I-1}) and writes H_I.
+// The grid is a multiple of the XCD count (k_sep_stage splits its windows
+// per XCD).
+template <int K, int W, int I>
+int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
+    if constexpr (I > K) {
+        return 0;
+    } else {
+        SepStage st{};
+        st.r = I == 1 ? 1u : s->tile_rows;
+        // LDS of a tile: image, base[n] and n * r rows; fewer rows per tile (or
+        // the row kernel) where a large graph image leaves too little room
+        auto tile_lds = [&](uint32_t r) { return ((s->gv.image_bytes + 15) & ~15u) + 2 * s->g.n * r * W * 8; };
+        auto fits = [&](uint32_t r) {
+            return tile_lds(r) <= kLdsBytes && s->g.n * r * W <= static_cast<uint32_t>(kTileLoads * kBlock);
+        };
+        while (st.r > 1 && !fits(st.r)) --st.r;
+        const bool tiles = s->pull_tiles && fits(st.r);
+        auto kern = tiles ? k_pull_tile<I, K - I, W> : k_sep_stage<I, K - I, W>;
+        const void* fn = reinterpret_cast<const void*>(kern);
+        st.stage = I;
+        st.in = in;
+        st.out = I == K ? s->d_nx0 : s->d_sep[(I - 1) & 1];
+        st.cnt_b = static_cast<uint32_t>(s->ms_count[K - I]);
+        st.cnt_b_in = static_cast<uint32_t>(s->ms_count[K - I + 1]);
+        st.cnt_a = static_cast<uint32_t>(s->ms_count[I]);
+        st.cnt_a_in = static_cast<uint32_t>(s->ms_count[I - 1]);
+        st.rows = s->ms_count[I] * s->ms_count[K - I];
+        uint32_t lds = s->gv.image_bytes;
+        uint64_t blocks = 0;
+        if (tiles) {
+            lds = tile_lds(st.r);
+            const uint64_t ntiles = static_cast<uint64_t>(st.cnt_b) * ((st.cnt_a_in + st.r - 1) / st.r);
+            blocks = std::min<uint64_t>(ntiles, s->tile_blocks);
+        } else {
+            blocks = std::min<uint64_t>(((st.rows + 63) / 64 + kWavesPerBlock - 1) / kWavesPerBlock, 16384);
+        }
+        if (int rc = raise_lds_limit(fn, lds)) return rc;
+        blocks = std::max<uint64_t>(kXcds, (blocks + kXcds - 1) / kXcds * kXcds);
+        Timer* t = start_timer(s, "sep_stage");
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), lds, s->stream, s->gv, a, st);
+        t->stop();
+        return launch_pull_stage<K, W, I + 1>(s, a, st.out);
+    }
+}
+
 template <int K, int W>
 int launch_pull(cr_solver* s, LevelArgs a) {
-    if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_sep_stage<K, W>), s->gv.image_bytes)) return rc;
-    const uint64_t* in = s->d_fr1;
-    for (int i = 1; i <= K; ++i) {
-        SepStage st{};
-        st.stage = i;
-        st.in = in;
-        st.out = i == K ? s->d_nx0 : s->d_sep[(i - 1) & 1];
-        st.cnt_b = static_cast<uint32_t>(s->ms_count[K - i]);
-        st.cnt_b_in = static_cast<uint32_t>(s->ms_count[K - i + 1]);
-        st.rows = s->ms_count[i] * s->ms_count[K - i];
-        const uint64_t blocks = std::min<uint64_t>((st.rows + kBlock - 1) / kBlock, 16384);
-        Timer* t = start_timer(s, "sep_stage");
-        hipLaunchKernelGGL((k_sep_stage<K, W>), dim3(std::max<uint64_t>(blocks, 1)), dim3(kBlock), s->gv.image_bytes,
-                           s->stream, s->gv, a, st);
-        t->stop();
-        in = st.out;
-    }
+    if (int rc = launch_pull_stage<K, W, 1>(s, a, s->d_fr1)) return rc;
     Timer* t = start_timer(s, "clear_frontier");
     hipLaunchKernelGGL(k_clear_frontier, dim3(1024), dim3(kBlock), 0, s->stream, a, s->W);
     t->stop();
@@ -2255,6 +2558,15 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             if (value < 0) return fail(CR_ERR_ARG, "pull scale %lld", static_cast<long long>(value));
             s->pull_scale = static_cast<uint64_t>(value);
             set_pull_threshold(s);
+            return 0;
+        case CR_TUNE_PULL_TILES: s->pull_tiles = value != 0; return 0;
+        case CR_TUNE_PULL_TILE_BLOCKS:
+            if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "tile blocks %lld", static_cast<long long>(value));
+            s->tile_blocks = static_cast<uint32_t>(value);
+            return 0;
+        case CR_TUNE_PULL_TILE_ROWS:
+            if (value < 1 || value > kTileMaxRows) return fail(CR_ERR_ARG, "tile rows %lld", static_cast<long long>(value));
+            s->tile_rows = static_cast<uint32_t>(value);
             return 0;
         case CR_TUNE_PRED_CSR: s->use_csr = value != 0; if (s->stage > kLoaded) s->stage = kLoaded; return 0;
         case CR_TUNE_LEVELS_PER_SYNC:

@@ -224,9 +224,13 @@ enum cr_tuning {
     CR_TUNE_ROW_BLOCKS = 3,     /* workgroups of the row kernels (default 2048) */
     CR_TUNE_LEVELS_PER_SYNC = 4, /* depth levels enqueued per host synchronisation (default 8) */
     CR_TUNE_PRED_CSR = 5,        /* 1: cr_build_aux_graph precomputes predecessor lists (default 0) */
-    CR_TUNE_PULL_SCALE = 6       /* a level gathers (factorised pull) instead of pushing when its robber
+    CR_TUNE_PULL_SCALE = 6,      /* a level gathers (factorised pull) instead of pushing when its robber
                                     frontier exceeds value/1000 times the estimated break-even row count;
                                     0: always pull, >= 10^9: never (default 500) */
+    CR_TUNE_PULL_TILES = 7,      /* 1 (default): the pull's stages load their input rows through LDS
+                                    tiles; 0: one row per lane with direct gathers */
+    CR_TUNE_PULL_TILE_ROWS = 8,  /* rows of each input block per LDS tile, 1..16 (default 2) */
+    CR_TUNE_PULL_TILE_BLOCKS = 9 /* workgroups of a tiled pull stage (default 8192) */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:

@@ -194,3 +194,46 @@ def test_repeat_solves_are_identical(solver):
     solver.initialize_captures()
     solver.main_loop()
     assert np.array_equal(solver.states(), a)
+
+
+PULL_SETTINGS = [{"pull_scale": 0, "pull_tiles": 1, "pull_tile_rows": 2},
+                 {"pull_scale": 0, "pull_tiles": 1, "pull_tile_rows": 3},
+                 {"pull_scale": 0, "pull_tiles": 0},
+                 {"pull_scale": 10**9}]
+PULL_IDS = ["pull_tiles2", "pull_tiles3", "pull_rows", "push_only"]
+PULL_CASES = ["v1__cycle100__k2__p1", "v1__dodecahedron__k3__p1", "v1__grid36__k3__p1", "v1__grid9__k3__p1",
+              "v1__line3__k2__p1", "v1__outerplanar8__k3__p1", "v1__peterson__k3__p1", "v1__planar32__k3__p1",
+              "v1__scotlandyard-all__k2__p1", "v1__test__k2__p1"]
+
+
+@pytest.mark.parametrize("setting", PULL_SETTINGS, ids=PULL_IDS)
+def test_pull_kernels_and_push_agree_with_reference(solver, setting):
+    """Every level by the factorised pull (LDS-tiled stages with tiles of
+    several sizes, or one row per lane) or every level by the push gives the
+    reference's tables and pass counts."""
+    try:
+        solver.set_tuning(**setting)
+        for name in PULL_CASES:
+            e = INDEX[name]
+            say("direction", setting, name)
+            _, passes, _ = run_steps(solver, matrix(e["graph"]), e["k"])
+            assert hashlib.sha256(solver.states().tobytes()).hexdigest() == e["states_sha256"], name
+            assert passes == e["passes"], name
+    finally:
+        solver.set_tuning(pull_scale=500, pull_tiles=1, pull_tile_rows=2)
+
+
+@pytest.mark.parametrize("setting", PULL_SETTINGS[:3], ids=PULL_IDS[:3])
+def test_pull_stages_for_four_to_six_cops_match_oracle(solver, oracle, setting):
+    """k = 4..6 run pull stages with up to five cops still to move; the tables
+    equal the oracle's."""
+    try:
+        solver.set_tuning(**setting)
+        for graph, k in (("planar8", 4), ("peterson", 4), ("outerplanar8", 5), ("line10", 5), ("grid9", 6)):
+            say("pull k", setting, graph, k)
+            n, adj = oracle.load_matrix(matrix(graph))
+            run_steps(solver, matrix(graph), k)
+            want = oracle.v1_solve(adj, k, 1)
+            assert np.array_equal(solver.states(), want.states), (graph, k)
+    finally:
+        solver.set_tuning(pull_scale=500, pull_tiles=1, pull_tile_rows=2)
