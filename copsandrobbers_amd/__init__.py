@@ -40,7 +40,7 @@ EXPORTS = (
     "cr_get_info", "cr_get_stats", "cr_copy_states", "cr_copy_configs", "cr_level_counts",
     "cr_count_retro_edges", "cr_work_profile", "cr_get_macro_stats", "cr_write_dp_table", "cr_extract_game",
     "cr_write_perfect_game", "cr_cop_number", "cr_count_transitions",
-    "cr_set_tuning", "cr_set_kernel_timing", "cr_kernel_times", "cr_cache_key",
+    "cr_set_tuning", "cr_set_kernel_timing", "cr_set_kernel_timing_filter", "cr_kernel_times", "cr_cache_key",
     "cr_batch_create", "cr_batch_destroy", "cr_batch_solve", "cr_batch_set_tuning", "cr_batch_pass_counts",
     "cr_batch_last_timing", "cr_server_set_tuning",
     "cr_server_create", "cr_server_submit", "cr_server_submit_many", "cr_server_poll", "cr_server_stats", "cr_server_destroy",
@@ -139,6 +139,7 @@ def lib():
         L.cr_get_macro_stats.argtypes = [vp, ctypes.POINTER(MacroStats), u64p, u64p, i32]
         L.cr_set_tuning.argtypes = [vp, i32, ctypes.c_int64]
         L.cr_set_kernel_timing.argtypes = [vp, i32]
+        L.cr_set_kernel_timing_filter.argtypes = [vp, ctypes.c_char_p]
         L.cr_kernel_times.argtypes = [vp, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int64), i32, ctypes.POINTER(i32)]
         L.cr_cache_key.argtypes = [ctypes.c_char_p, i32, i32, i32, ctypes.c_char_p, i32]
@@ -354,7 +355,10 @@ class Solver:
         for name, value in knobs.items():
             _check(lib().cr_set_tuning(self._h, TUNING[name], int(value)))
 
-    def set_kernel_timing(self, enabled=True):
+    def set_kernel_timing(self, enabled=True, only=None):
+        """Per-launch HIP-event timing of the main loop's kernels; `only` names
+        the one kernel to time (None: all)."""
+        _check(lib().cr_set_kernel_timing_filter(self._h, (only or "").encode()))
         _check(lib().cr_set_kernel_timing(self._h, 1 if enabled else 0))
 
     def kernel_times(self):

@@ -149,6 +149,7 @@ template <int K, int W>
 __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint64_t lds_om[CR_MAX_N * W];
+    __shared__ uint64_t stage[kWavesPerBlock][3][W][64];  // per wave: its rows' cm, c1, r1 words
     load_image(lds, gv.image, gv.image_bytes);
     const uint64_t* lom = stage_out_masks<W>(lds_om, a.out_mask, gv.n);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
@@ -192,12 +193,21 @@ __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
             }
             bool has0 = false, has1 = false;
             uint32_t lc = 0, l0 = 0, l1 = 0;
+            uint64_t c1[W];
+#pragma unroll
+            for (int q = 0; q < W; ++q) c1[q] = am[q] & ~cm[q];
+            // table bytes of position rb: (cap 0x01 | depth-1 0x03 | unmarked 0xFE) per turn
+            auto pos_pair = [&](uint64_t cw, uint64_t c1w, uint64_t r1w, int rb) -> uint32_t {
+                const uint64_t bit = 1ull << rb;
+                const bool cap = cw & bit;
+                const uint32_t b0 = cap ? 0x01u : ((c1w & bit) ? 0x03u : kUnmarked);
+                const uint32_t b1 = cap ? 0x01u : ((r1w & bit) ? 0x03u : kUnmarked);
+                return b0 | (b1 << 8);
+            };
             if (live) {
                 const uint64_t o = static_cast<uint64_t>(row) * W;
-                uint64_t c1[W];
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
-                    c1[q] = am[q] & ~cm[q];
                     a.won0[o + q] = cm[q] | c1[q];
                     a.won1[o + q] = cm[q] | r1[q];
                     a.fr1[o + q] = r1[q];
@@ -208,15 +218,39 @@ __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
                     has0 |= c1[q] != 0;
                     has1 |= r1[q] != 0;
                 }
-                // the row's 2N table bytes: (cap 0x01 | depth-1 0x03 | unmarked 0xFE) per turn
+            }
+            if ((n & 7) == 0) {
+                // the wave's rows are one contiguous span of 64 * 2n bytes: lanes
+                // write consecutive 16-byte chunks (8 positions) of it, reading the
+                // owning row's masks from the wave's LDS slot
+#pragma unroll
+                for (int q = 0; q < W; ++q) {
+                    stage[wid][0][q][lane] = cm[q];
+                    stage[wid][1][q][lane] = c1[q];
+                    stage[wid][2][q][lane] = r1[q];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const uint32_t rows_here = min(64u, gv.C - b * 64);
+                const uint32_t cpr = static_cast<uint32_t>(n) / 8;  // chunks per row
+                uint4* dst = reinterpret_cast<uint4*>(a.table + static_cast<uint64_t>(b) * 64 * n * 2);
+                for (uint32_t i = lane; i < rows_here * cpr; i += 64) {
+                    const uint32_t rl = i / cpr, r0 = (i - rl * cpr) * 8;
+                    const int q = r0 >> 6, sh = r0 & 63;
+                    const uint64_t cw = stage[wid][0][q][rl] >> sh, c1w = stage[wid][1][q][rl] >> sh,
+                                   r1w = stage[wid][2][q][rl] >> sh;
+                    uint32_t d[4];
+#pragma unroll
+                    for (int h = 0; h < 4; ++h)
+                        d[h] = pos_pair(cw, c1w, r1w, 2 * h) | (pos_pair(cw, c1w, r1w, 2 * h + 1) << 16);
+                    dst[i] = make_uint4(d[0], d[1], d[2], d[3]);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            } else if (live) {
                 uint8_t* trow = a.table + static_cast<uint64_t>(row) * n * 2;
-                auto pos_pair = [&](uint64_t cw, uint64_t c1w, uint64_t r1w, int rb) -> uint32_t {
-                    const uint64_t bit = 1ull << rb;
-                    const bool cap = cw & bit;
-                    const uint32_t b0 = cap ? 0x01u : ((c1w & bit) ? 0x03u : kUnmarked);
-                    const uint32_t b1 = cap ? 0x01u : ((r1w & bit) ? 0x03u : kUnmarked);
-                    return b0 | (b1 << 8);
-                };
 #pragma unroll
                 for (int w = 0; w < W; ++w) {
                     const int rend = min(64, n - 64 * w);
@@ -1301,6 +1335,7 @@ struct cr_solver {
     std::map<std::string, KTime> ktimes;
     std::vector<std::string> korder;
     std::vector<hipEvent_t> events;  // reusable timing events
+    std::string time_only;           // when set, only launches of this kernel are timed
 
     ~cr_solver() { release(); }
 
@@ -1407,20 +1442,22 @@ struct Timer {
     cr_solver* s;
     const char* name;
     hipEvent_t a = nullptr, b = nullptr;
+    bool on = false;
     Timer(cr_solver* s_, const char* n) : s(s_), name(n) {
-        if (s->timing) {
+        on = s->timing && (s->time_only.empty() || s->time_only == n);
+        if (on) {
             a = acquire_event(s);
             b = acquire_event(s);
             hipEventRecord(a, s->stream);
         }
     }
     void stop() {
-        if (!s->timing) return;
+        if (!on) return;
         hipEventRecord(b, s->stream);
     }
     // called after the stream has been synchronised
     void collect() {
-        if (!s->timing || a == nullptr) return;
+        if (!on || a == nullptr) return;
         float ms = 0;
         hipEventElapsedTime(&ms, a, b);
         auto it = s->ktimes.find(name);
@@ -2575,6 +2612,12 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             return 0;
         default: return fail(CR_ERR_ARG, "unknown tuning key %d", key);
     }
+}
+
+int cr_set_kernel_timing_filter(cr_solver* s, const char* name) {
+    if (int rc = check_solver(s)) return rc;
+    s->time_only = name ? name : "";
+    return 0;
 }
 
 int cr_set_kernel_timing(cr_solver* s, int32_t enabled) {

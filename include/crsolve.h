@@ -236,6 +236,10 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:
  * name_out receives a '\0'-separated list, ms/launches one entry per name. */
 int cr_set_kernel_timing(cr_solver* s, int32_t enabled);
+/* Restricts timing to the launches of one kernel (name as cr_kernel_times
+ * reports it); NULL or "" times every launch. Timing events sit between
+ * launches, so timing fewer kernels perturbs the loop less. */
+int cr_set_kernel_timing_filter(cr_solver* s, const char* name);
 int cr_kernel_times(cr_solver* s, char* names_out, int32_t names_cap, double* ms,
                     int64_t* launches, int32_t cap, int32_t* count);
 /* Build-time cache key, CacheManager::buildCacheKey (CacheManager.h:16-26),
