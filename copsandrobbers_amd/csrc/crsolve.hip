@@ -76,6 +76,14 @@ struct Ctl {
     uint32_t cnt1[2];  // ... with new robber-turn states
     uint32_t pull;     // the level's cop turn gathers (pull) instead of pushing
     uint32_t dir[128]; // per level: 1 = pulled
+    // depth log: the lists of new states of each depth with their robber
+    // bits, applied to the table by k_apply_log (see there)
+    uint32_t log_now;               // the current depth's lists go to the log (else straight to the table)
+    uint32_t log_open;              // the lists of the depth that ended the loop are still to be written
+    unsigned long long log_used;    // entries in use
+    unsigned long long log_cap;     // capacity in entries (0: no log)
+    unsigned long long log_off[128][2];  // per depth and turn: first entry
+    uint32_t log_len[128][2];            // per depth and turn: entries (0 when written directly)
     unsigned long long lvl[128][2];
     unsigned long long hist[2][128];  // retro-edges by depth: [0] cop (robber states), [1] robber (cop states)
     unsigned long long verdict;
@@ -113,6 +121,9 @@ struct LevelArgs {
     uint32_t nbatch;           // ceil(C / 64)
     int32_t filter;
     uint32_t pull_rows;        // a level pulls when its robber frontier has more rows than this
+    uint32_t* log_rows;        // depth log: configuration of each entry
+    uint64_t* log_bits;        // depth log: W robber words of each entry
+    int32_t n, W;
 };
 
 // Work items [0, n) in 8 contiguous segments, one per XCD (workgroups are
@@ -789,7 +800,6 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
     const uint32_t nwords = (count + 63) / 64;
     const uint32_t ngroups = (nwords + kGroupWords - 1) / kGroupWords;
     const uint32_t wid = wave_id();
-    const uint8_t nb = won_byte(level + 1);
     const int n = gv.n;
     for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
         unsigned long long rows = 0, found = 0;
@@ -826,13 +836,7 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
                     a.won1[o + q] = w1[q] | nw[q];
-                    a.fr1[o + q] = nw[q];
-                    uint64_t bits = nw[q];
-                    while (bits) {
-                        const int r = q * 64 + __builtin_ctzll(bits);
-                        bits &= bits - 1;
-                        a.table[(static_cast<uint64_t>(row) * n + r) * 2 + 1] = nb;
-                    }
+                    a.fr1[o + q] = nw[q];  // the depth bytes follow from list1 (k_write_lists)
                 }
             }
             const uint64_t f = __ballot(append);
@@ -856,8 +860,6 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs a) {
     const int lane = threadIdx.x & 63;
     if (a.ctl->done) return;
-    const uint32_t level = a.ctl->level;
-    const uint8_t nb = won_byte(level + 1);
     const uint32_t ngroups = (a.nbatch + kGroupWords - 1) / kGroupWords;
     const uint32_t wid = wave_id();
     for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
@@ -881,14 +883,10 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
                     const uint64_t wv = a.won0[o + q];
                     const uint64_t nw = x[q] & ~wv;
                     pop += __popcll(nw);
-                    a.nx0[o + q] = 0;
+                    // the new states stay in nx0 until k_write_lists records their
+                    // depth bytes and clears the row
+                    a.nx0[o + q] = nw;
                     if (nw) a.won0[o + q] = wv | nw;
-                    uint64_t bits = nw;
-                    while (bits) {
-                        const int r = q * 64 + __builtin_ctzll(bits);
-                        bits &= bits - 1;
-                        a.table[(static_cast<uint64_t>(row) * gv.n + r) * 2] = nb;
-                    }
                 }
             }
             const uint64_t f = __ballot(pop != 0);
@@ -950,7 +948,10 @@ __device__ void scan_groups(uint32_t* gcnt, const unsigned long long* gstates, u
 // reach the 127 cap (levels run d = 1..125).
 __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
     Ctl* ctl = a.ctl;
-    if (ctl->done) return;
+    if (ctl->done) {  // a queued level after the last one: its k_write_lists does nothing
+        if (threadIdx.x == 0) ctl->log_open = 0;
+        return;
+    }
     const uint32_t d = ctl->level, cur = d & 1, nxt = cur ^ 1;
     const uint32_t ng0 = (a.nbatch + kGroupWords - 1) / kGroupWords;
     const uint32_t nw1 = (ctl->cnt0[cur] + 63) / 64;
@@ -967,7 +968,18 @@ __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
         ctl->pull = choose_pull(t1, a.pull_rows);
         ctl->dir[d + 1] = ctl->pull;
         ctl->level = d + 1;
-        if ((t0 == 0 && t1 == 0) || d + 1 >= kMaxRound - 1) ctl->done = 1;
+        // room in the depth log for the lists of depth d + 1, or direct writes
+        const unsigned long long need = static_cast<unsigned long long>(t0) + t1;
+        ctl->log_now = ctl->log_cap != 0 && ctl->log_used + need <= ctl->log_cap;
+        ctl->log_off[d + 1][0] = ctl->log_used;
+        ctl->log_off[d + 1][1] = ctl->log_used + t0;
+        ctl->log_len[d + 1][0] = ctl->log_now ? t0 : 0u;
+        ctl->log_len[d + 1][1] = ctl->log_now ? t1 : 0u;
+        if (ctl->log_now) ctl->log_used += need;
+        if ((t0 == 0 && t1 == 0) || d + 1 >= kMaxRound - 1) {
+            ctl->done = 1;
+            ctl->log_open = 1;  // the depth d + 1 lists still carry their states' bytes
+        }
     }
 }
 
@@ -1011,7 +1023,9 @@ __global__ __launch_bounds__(kScanThreads) void k_settle_init(LevelArgs a) {
 // (both spaces are configurations, lists go to parity 1).
 __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t init_mode) {
     const Ctl* ctl = a.ctl;
-    if (ctl->done) return;  // no further level reads the lists
+    // no further level reads the lists; only the depth that ended the loop
+    // still has its states' bytes to record
+    if (ctl->done && (init_mode || !ctl->log_open)) return;
     // after k_advance, level already names the next depth: its parity holds the new lists
     const uint32_t nxt = init_mode ? 1u : (ctl->level & 1);
     const uint32_t prev = nxt ^ 1;
@@ -1040,6 +1054,193 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
             rest &= rest - 1;
             dst[pos++] = (space1 && !init_mode) ? a.list0[prev][idx] : idx;
         }
+    }
+}
+
+// The depth bytes of the states listed by k_write_lists (not in init mode):
+// one thread per list entry takes the row's new robber bits (fr1 for the
+// robber turn; nx0 for the cop turn, left there by k_cop_finalize and cleared
+// here) into the depth log, or, when the log has no room for this depth,
+// writes the bytes into the table directly.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
+    const Ctl* ctl = a.ctl;
+    if (ctl->done && !ctl->log_open) return;
+    const uint32_t depth = ctl->level, nxt = depth & 1;
+    const uint32_t n0 = ctl->cnt0[nxt], n1 = ctl->cnt1[nxt];
+    const bool logging = ctl->log_now != 0;
+    const uint8_t nb = won_byte(depth);
+    const unsigned long long off0 = ctl->log_off[depth][0], off1 = ctl->log_off[depth][1];
+    const uint64_t total = static_cast<uint64_t>(n0) + n1;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const bool robber = i >= n0;
+        const uint32_t k = static_cast<uint32_t>(robber ? i - n0 : i);
+        const uint32_t row = (robber ? a.list1[nxt] : a.list0[nxt])[k];
+        uint64_t* src = (robber ? a.fr1 : a.nx0) + static_cast<uint64_t>(row) * W;
+        uint64_t w[W];
+        load_row<W>(src, w);
+        if (!robber) {
+            uint64_t z[W];
+#pragma unroll
+            for (int q = 0; q < W; ++q) z[q] = 0;
+            store_row<W>(src, z);
+        }
+        if (logging) {
+            const unsigned long long e = (robber ? off1 : off0) + k;
+            a.log_rows[e] = row;
+            store_row<W>(a.log_bits + e * W, w);
+        } else {
+            uint8_t* trow = a.table + static_cast<uint64_t>(row) * a.n * 2 + (robber ? 1 : 0);
+#pragma unroll
+            for (int q = 0; q < W; ++q) {
+                uint64_t x = w[q];
+                while (x) {
+                    trow[2 * (q * 64 + __builtin_ctzll(x))] = nb;
+                    x &= x - 1;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// depth log -> table. The levels record each depth's new states as sorted
+// lists of (configuration, robber bits); writing their depth bytes straight
+// into the table would touch a few bytes of many 32-byte sectors of a row at
+// every depth. Instead the table is taken in chunks of R rows through LDS
+// (coalesced), every logged depth's entries for the chunk are applied there,
+// and the chunk is written back once. k_log_starts first finds, for every
+// chunk and list, the list's first entry in the chunk.
+// ---------------------------------------------------------------------------
+struct LogList {
+    unsigned long long off;    // first entry in the log
+    unsigned long long first;  // entries of the preceding lists (flat index of entry 0)
+    uint32_t len;              // entries
+    uint32_t turn_byte;        // turn << 8 | depth byte
+};
+
+// Per chunk c and list L: cnt[c * nl + L] entries of L have rows in chunk c,
+// the first at beg[c * nl + L] (cnt is zeroed before; beg is only read where
+// cnt > 0). One thread per entry; the first entry of a chunk finds the chunk's
+// last one by binary search over at most R entries.
+__global__ __launch_bounds__(kBlock) void k_log_chunks(const LogList* lists, int32_t nl, uint64_t total,
+                                                       const uint32_t* rows, uint32_t* beg, uint32_t* cnt,
+                                                       uint32_t R) {
+    __shared__ LogList sl[256];
+    for (int L = threadIdx.x; L < nl; L += blockDim.x) sl[L] = lists[L];
+    __syncthreads();
+    for (uint64_t g = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; g < total;
+         g += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        int lo = 0, hi = nl - 1;  // the list holding flat entry g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sl[mid].first <= g) lo = mid; else hi = mid - 1;
+        }
+        const LogList ll = sl[lo];
+        const uint32_t i = static_cast<uint32_t>(g - ll.first);
+        const uint32_t* lr = rows + ll.off;
+        const uint32_t c = lr[i] / R;
+        if (i > 0 && lr[i - 1] / R == c) continue;  // not the chunk's first entry
+        uint32_t a = i + 1, b = min(ll.len, i + R);  // first entry past the chunk lies in [a, b]
+        while (a < b) {
+            const uint32_t mid = (a + b) >> 1;
+            if (lr[mid] / R == c) a = mid + 1; else b = mid;
+        }
+        beg[static_cast<uint64_t>(c) * nl + lo] = i;
+        cnt[static_cast<uint64_t>(c) * nl + lo] = a - i;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+// Exclusive scan of one value per thread over the block (kBlock threads);
+// returns the thread's prefix and the block total in `total`.
+__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t wsum[kWavesPerBlock];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t base = 0;
+    total = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+        if (w < wid) base += wsum[w];
+        total += wsum[w];
+    }
+    __syncthreads();
+    return base + incl - v;
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_apply_log(const LogList* lists, int32_t nl, const uint32_t* rows,
+                                                      const uint64_t* bits, const uint32_t* beg, const uint32_t* cnt,
+                                                      uint8_t* table, uint32_t n, uint32_t C, uint32_t R) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint32_t s_beg[256], s_pre[256];
+    __shared__ unsigned long long s_off[256];
+    __shared__ uint32_t s_tb[256];
+    const uint32_t rowbytes = 2 * n;
+    const uint32_t nchunks = (C + R - 1) / R;
+    for (int L = threadIdx.x; L < nl; L += blockDim.x) {
+        s_off[L] = lists[L].off;
+        s_tb[L] = lists[L].turn_byte;
+    }
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const uint32_t R0 = ch * R, R1 = min(R0 + R, C);
+        __syncthreads();  // the previous chunk is written back
+        uint32_t c = 0;
+        if (threadIdx.x < static_cast<uint32_t>(nl)) {
+            c = cnt[static_cast<uint64_t>(ch) * nl + threadIdx.x];
+            if (c) s_beg[threadIdx.x] = beg[static_cast<uint64_t>(ch) * nl + threadIdx.x];
+        }
+        uint32_t total = 0;
+        const uint32_t pre = block_scan_u32(c, total);
+        if (threadIdx.x < static_cast<uint32_t>(nl)) s_pre[threadIdx.x] = pre;
+        if (total == 0) continue;  // uniform
+        // the chunk's rows into LDS: 16-byte words (R is a multiple of 8, so
+        // R0 * 2n is a multiple of 16), then the tail bytes
+        uint8_t* trow = table + static_cast<uint64_t>(R0) * rowbytes;
+        const uint32_t nbytes = (R1 - R0) * rowbytes, nvec = nbytes / 16;
+        for (uint32_t i = threadIdx.x; i < nvec; i += blockDim.x)
+            reinterpret_cast<uint4*>(lds)[i] = reinterpret_cast<const uint4*>(trow)[i];
+        for (uint32_t i = nvec * 16 + threadIdx.x; i < nbytes; i += blockDim.x) lds[i] = trow[i];
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+            int lo = 0, hi = nl - 1;  // the list holding the chunk's entry e (the last with prefix <= e)
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= e) lo = mid; else hi = mid - 1;
+            }
+            const unsigned long long ent = s_off[lo] + s_beg[lo] + (e - s_pre[lo]);
+            const uint32_t row = rows[ent];
+            uint64_t w[W];
+            load_row<W>(bits + ent * W, w);
+            const uint32_t tb = s_tb[lo];
+            uint8_t* rp = lds + (row - R0) * rowbytes + (tb >> 8);
+            const uint8_t byte = static_cast<uint8_t>(tb & 0xFF);
+#pragma unroll
+            for (int q = 0; q < W; ++q) {
+                uint64_t x = w[q];
+                while (x) {
+                    const uint32_t r = q * 64 + __builtin_ctzll(x);
+                    x &= x - 1;
+                    if (r < n) rp[2 * r] = byte;
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nvec; i += blockDim.x)
+            reinterpret_cast<uint4*>(trow)[i] = reinterpret_cast<const uint4*>(lds)[i];
+        for (uint32_t i = nvec * 16 + threadIdx.x; i < nbytes; i += blockDim.x) trow[i] = lds[i];
     }
 }
 
@@ -1283,6 +1484,15 @@ struct cr_solver {
     uint64_t* d_fr1 = nullptr;
     uint64_t* d_nx0 = nullptr;
     uint32_t* d_list[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [turn][parity] dense row lists
+    // depth log (k_write_lists, k_apply_log)
+    uint32_t* d_log_rows = nullptr;
+    uint64_t* d_log_bits = nullptr;
+    uint64_t log_cap = 0;          // entries; 0: the lists write their depth bytes directly
+    uint32_t log_factor = 24;      // tuning: log capacity in units of C entries (0: no log)
+    LogList* d_lists = nullptr;    // descriptors of the depths k_apply_log applies
+    uint32_t* d_starts = nullptr;  // k_log_chunks: per chunk and list, first entry and count
+    LogList* h_lists = nullptr;    // pinned staging of the descriptors
+    uint32_t log_first = 0;        // first depth whose lists are not in the table yet
     uint64_t* d_flags[2] = {nullptr, nullptr};                          // producer flag words per turn
     uint32_t* d_gcnt[2] = {nullptr, nullptr};                           // per-group counts / offsets
     unsigned long long* d_gstates[2] = {nullptr, nullptr};              // per-group state counts
@@ -1364,6 +1574,15 @@ struct cr_solver {
         hipFree(d_sep[0]);
         hipFree(d_sep[1]);
         d_sep[0] = d_sep[1] = nullptr;
+        hipFree(d_log_rows);
+        hipFree(d_log_bits);
+        hipFree(d_lists);
+        hipFree(d_starts);
+        d_starts = nullptr;
+        d_log_rows = nullptr;
+        d_log_bits = nullptr;
+        d_lists = nullptr;
+        log_cap = 0;
         csr_entries = 0;
         crd::macro_free(macro, device_bytes);
         macro = nullptr;
@@ -1635,6 +1854,10 @@ LevelArgs level_args(cr_solver* s) {
     a.fr1 = s->d_fr1;
     a.nx0 = s->d_nx0;
     a.table = s->d_table;
+    a.log_rows = s->d_log_rows;
+    a.log_bits = s->d_log_bits;
+    a.n = s->g.n;
+    a.W = s->W;
     for (int b = 0; b < 2; ++b) {
         a.list0[b] = s->d_list[0][b];
         a.list1[b] = s->d_list[1][b];
@@ -1774,6 +1997,99 @@ int alloc_pull(cr_solver* s) {
     return 0;
 }
 
+// Depth log: up to log_factor * C entries of (configuration, W robber
+// words), within half of the free device memory (none below 2 * C entries).
+// A depth whose lists do not fit in the remaining room writes its bytes
+// directly.
+constexpr int kMaxLogLists = 256;
+uint32_t log_chunk_rows(const cr_solver* s);
+
+int alloc_log(cr_solver* s) {
+    if (!s->log_factor) return 0;
+    const uint64_t entry = 4 + 8ull * s->W;
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    const uint64_t cap = std::min<uint64_t>(static_cast<uint64_t>(s->log_factor) * s->C, freeb / 2 / entry);
+    if (cap < 2 * s->C) return 0;
+    int rc = 0;
+    if (!(s->d_log_rows = dev_alloc<uint32_t>(cap, s->device_bytes, rc))) return rc;
+    if (!(s->d_log_bits = dev_alloc<uint64_t>(cap * s->W, s->device_bytes, rc))) return rc;
+    if (!(s->d_lists = dev_alloc<LogList>(kMaxLogLists, s->device_bytes, rc))) return rc;
+    const uint64_t nchunks = (s->C + log_chunk_rows(s) - 1) / log_chunk_rows(s);
+    if (!(s->d_starts = dev_alloc<uint32_t>(2 * nchunks * kMaxLogLists, s->device_bytes, rc))) return rc;
+    if (!s->h_lists)
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_lists), kMaxLogLists * sizeof(LogList), hipHostMallocDefault));
+    s->log_cap = cap;
+    return 0;
+}
+
+// Rows per chunk of k_apply_log (even, 48 KiB of LDS).
+uint32_t log_chunk_rows(const cr_solver* s) {
+    return std::max<uint32_t>(8, std::min<uint32_t>(8192, 48 * 1024 / (2 * s->g.n))) & ~7u;
+}
+
+template <int W>
+struct LaunchApplyLog {
+    static int run(cr_solver* s, int32_t nl, uint64_t total) {
+        const uint32_t n = s->g.n, R = log_chunk_rows(s);
+        const uint32_t lds = R * 2 * n;
+        const uint32_t nchunks = static_cast<uint32_t>((s->C + R - 1) / R);
+        const uint64_t cells = static_cast<uint64_t>(nchunks) * nl;
+        uint32_t* beg = s->d_starts;
+        uint32_t* cnt = s->d_starts + static_cast<uint64_t>(nchunks) * kMaxLogLists;
+        hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<uint32_t>(std::min<uint64_t>((cells + kBlock - 1) / kBlock, 4096))),
+                           dim3(kBlock), 0, s->stream, cnt, cells, 0u);
+        const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((total + kBlock - 1) / kBlock, 8192)));
+        hipLaunchKernelGGL(k_log_chunks, dim3(sgrid), dim3(kBlock), 0, s->stream, s->d_lists, nl, total, s->d_log_rows,
+                           beg, cnt, R);
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nchunks, 4096));
+        if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_apply_log<W>), lds)) return rc;
+        hipLaunchKernelGGL(k_apply_log<W>, dim3(grid), dim3(kBlock), lds, s->stream, s->d_lists, nl, s->d_log_rows,
+                           s->d_log_bits, beg, cnt, s->d_table, n, static_cast<uint32_t>(s->C), R);
+        return 0;
+    }
+};
+
+template <int W>
+struct LaunchLogLists {
+    static int run(cr_solver* s, LevelArgs a) {
+        hipLaunchKernelGGL(k_log_lists<W>, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, a);
+        return 0;
+    }
+};
+
+// Applies the logged depths [log_first, level] to the table and empties the
+// log, when the loop has ended (force) or the log may not hold the next
+// levels_per_sync depths of the size seen so far. h_ctl is current.
+int flush_log(cr_solver* s, bool force) {
+    if (!s->log_cap) return 0;
+    const Ctl* h = s->h_ctl;
+    const uint32_t last = std::min<uint32_t>(h->level, 127);
+    uint64_t peak = 0;
+    for (uint32_t d = s->log_first; d <= last; ++d)
+        peak = std::max<uint64_t>(peak, static_cast<uint64_t>(h->log_len[d][0]) + h->log_len[d][1]);
+    if (!force && h->log_used + std::max<uint64_t>(peak, 1) * s->levels_per_sync <= s->log_cap) return 0;
+    int32_t nl = 0;
+    uint64_t total = 0;
+    for (uint32_t d = s->log_first; d <= last; ++d)
+        for (int t = 0; t < 2; ++t)
+            if (h->log_len[d][t] && nl < kMaxLogLists) {
+                s->h_lists[nl++] = LogList{h->log_off[d][t], total, h->log_len[d][t],
+                                           static_cast<uint32_t>(t << 8) | static_cast<uint32_t>(1u | (d << 1))};
+                total += h->log_len[d][t];
+            }
+    if (nl) {
+        HIP_TRY(hipMemcpyAsync(s->d_lists, s->h_lists, nl * sizeof(LogList), hipMemcpyHostToDevice, s->stream));
+        Timer* t = start_timer(s, "apply_log");
+        if (int rc = dispatch_w<LaunchApplyLog>(s->W, s, nl, total)) return rc;
+        t->stop();
+    }
+    static const unsigned long long zero = 0;
+    HIP_TRY(hipMemcpyAsync(&s->d_ctl->log_used, &zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
+    s->log_first = last + 1;
+    return 0;
+}
+
 // Limited-visibility variant: one column of states, the marked plane (won0)
 // and the cop masks; the trajectory DAG is built by the main loop.
 int build_macro(cr_solver* s) {
@@ -1899,6 +2215,7 @@ int do_build(cr_solver* s) {
     if ((rc = upload_graph(s)) != 0) return rc;
     if (s->use_csr && (rc = build_csr(s)) != 0) return rc;
     if ((rc = alloc_pull(s)) != 0) return rc;
+    if ((rc = alloc_log(s)) != 0) return rc;
     s->stage = kBuilt;
     return 0;
 }
@@ -1907,6 +2224,8 @@ int do_build(cr_solver* s) {
 int reset_ctl(cr_solver* s) {
     std::memset(s->h_ctl, 0, sizeof(Ctl));
     s->h_ctl->verdict = ~0ull;
+    s->h_ctl->log_cap = s->log_cap;
+    s->log_first = 0;
     HIP_TRY(hipMemcpyAsync(s->d_ctl, s->h_ctl, sizeof(Ctl), hipMemcpyHostToDevice, s->stream));
     return 0;
 }
@@ -2095,13 +2414,17 @@ int do_loop(cr_solver* s) {
             t->stop();
             hipLaunchKernelGGL(k_advance, dim3(1), dim3(kScanThreads), 0, s->stream, a);
             hipLaunchKernelGGL(k_write_lists, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, a, 0);
+            if ((rc = dispatch_w<LaunchLogLists>(s->W, s, a)) != 0) return rc;
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl, offsetof(Ctl, lvl), hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
+        done = s->h_ctl->done != 0;
+        if ((rc = flush_log(s, done)) != 0) return rc;
+        HIP_TRY(hipGetLastError());
+        if (s->timing) HIP_TRY(hipStreamSynchronize(s->stream));
         for (auto& t : timers) t->collect();
         s->timers = nullptr;
-        done = s->h_ctl->done != 0;
     }
     HIP_TRY(hipEventRecord(e1, s->stream));
     HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
@@ -2237,6 +2560,7 @@ void cr_solver_destroy(cr_solver* s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     s->release();
     if (s->h_ctl) hipHostFree(s->h_ctl);
+    if (s->h_lists) hipHostFree(s->h_lists);
     for (hipEvent_t e : s->events) hipEventDestroy(e);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
@@ -2597,6 +2921,11 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             set_pull_threshold(s);
             return 0;
         case CR_TUNE_PULL_TILES: s->pull_tiles = value != 0; return 0;
+        case CR_TUNE_DEPTH_LOG:
+            if (value < 0 || value > 1024) return fail(CR_ERR_ARG, "depth log %lld", static_cast<long long>(value));
+            s->log_factor = static_cast<uint32_t>(value);
+            if (s->stage > kLoaded) s->stage = kLoaded;
+            return 0;
         case CR_TUNE_PULL_TILE_BLOCKS:
             if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "tile blocks %lld", static_cast<long long>(value));
             s->tile_blocks = static_cast<uint32_t>(value);

@@ -199,8 +199,11 @@ def test_repeat_solves_are_identical(solver):
 PULL_SETTINGS = [{"pull_scale": 0, "pull_tiles": 1, "pull_tile_rows": 2},
                  {"pull_scale": 0, "pull_tiles": 1, "pull_tile_rows": 3},
                  {"pull_scale": 0, "pull_tiles": 0},
-                 {"pull_scale": 10**9}]
-PULL_IDS = ["pull_tiles2", "pull_tiles3", "pull_rows", "push_only"]
+                 {"pull_scale": 10**9},
+                 {"depth_log": 0},
+                 {"depth_log": 2, "levels_per_sync": 3}]
+PULL_IDS = ["pull_tiles2", "pull_tiles3", "pull_rows", "push_only", "direct_bytes", "small_log"]
+DEFAULTS = {"pull_scale": 500, "pull_tiles": 1, "pull_tile_rows": 2, "depth_log": 24, "levels_per_sync": 8}
 PULL_CASES = ["v1__cycle100__k2__p1", "v1__dodecahedron__k3__p1", "v1__grid36__k3__p1", "v1__grid9__k3__p1",
               "v1__line3__k2__p1", "v1__outerplanar8__k3__p1", "v1__peterson__k3__p1", "v1__planar32__k3__p1",
               "v1__scotlandyard-all__k2__p1", "v1__test__k2__p1"]
@@ -209,7 +212,9 @@ PULL_CASES = ["v1__cycle100__k2__p1", "v1__dodecahedron__k3__p1", "v1__grid36__k
 @pytest.mark.parametrize("setting", PULL_SETTINGS, ids=PULL_IDS)
 def test_pull_kernels_and_push_agree_with_reference(solver, setting):
     """Every level by the factorised pull (LDS-tiled stages with tiles of
-    several sizes, or one row per lane) or every level by the push gives the
+    several sizes, or one row per lane), every level by the push, depth bytes
+    written directly, or through a depth log so small that it is applied at
+    every synchronisation and overflows into direct writes, gives the
     reference's tables and pass counts."""
     try:
         solver.set_tuning(**setting)
@@ -220,7 +225,7 @@ def test_pull_kernels_and_push_agree_with_reference(solver, setting):
             assert hashlib.sha256(solver.states().tobytes()).hexdigest() == e["states_sha256"], name
             assert passes == e["passes"], name
     finally:
-        solver.set_tuning(pull_scale=500, pull_tiles=1, pull_tile_rows=2)
+        solver.set_tuning(**DEFAULTS)
 
 
 @pytest.mark.parametrize("setting", PULL_SETTINGS[:3], ids=PULL_IDS[:3])
@@ -236,4 +241,4 @@ def test_pull_stages_for_four_to_six_cops_match_oracle(solver, oracle, setting):
             want = oracle.v1_solve(adj, k, 1)
             assert np.array_equal(solver.states(), want.states), (graph, k)
     finally:
-        solver.set_tuning(pull_scale=500, pull_tiles=1, pull_tile_rows=2)
+        solver.set_tuning(**DEFAULTS)
