@@ -801,6 +801,7 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
     const uint32_t ngroups = (nwords + kGroupWords - 1) / kGroupWords;
     const uint32_t wid = wave_id();
     const int n = gv.n;
+    const uint8_t nb = won_byte(level + 1);
     for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
         unsigned long long rows = 0, found = 0;
         for (int j = 0; j < kWordsPerWave; ++j) {
@@ -836,7 +837,14 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
                     a.won1[o + q] = w1[q] | nw[q];
-                    a.fr1[o + q] = nw[q];  // the depth bytes follow from list1 (k_write_lists)
+                    a.fr1[o + q] = nw[q];
+                    if (a.log_rows) continue;  // the depth bytes go through the log (k_log_lists)
+                    uint64_t bits = nw[q];
+                    while (bits) {
+                        const int r = q * 64 + __builtin_ctzll(bits);
+                        bits &= bits - 1;
+                        a.table[(static_cast<uint64_t>(row) * n + r) * 2 + 1] = nb;
+                    }
                 }
             }
             const uint64_t f = __ballot(append);
@@ -860,6 +868,7 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs a) {
     const int lane = threadIdx.x & 63;
     if (a.ctl->done) return;
+    const uint8_t nb = won_byte(a.ctl->level + 1);
     const uint32_t ngroups = (a.nbatch + kGroupWords - 1) / kGroupWords;
     const uint32_t wid = wave_id();
     for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
@@ -883,10 +892,20 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
                     const uint64_t wv = a.won0[o + q];
                     const uint64_t nw = x[q] & ~wv;
                     pop += __popcll(nw);
-                    // the new states stay in nx0 until k_write_lists records their
-                    // depth bytes and clears the row
-                    a.nx0[o + q] = nw;
                     if (nw) a.won0[o + q] = wv | nw;
+                    if (a.log_rows) {
+                        // the new states stay in nx0 until k_log_lists logs them and
+                        // clears the row
+                        a.nx0[o + q] = nw;
+                        continue;
+                    }
+                    a.nx0[o + q] = 0;
+                    uint64_t bits = nw;
+                    while (bits) {
+                        const int r = q * 64 + __builtin_ctzll(bits);
+                        bits &= bits - 1;
+                        a.table[(static_cast<uint64_t>(row) * gv.n + r) * 2] = nb;
+                    }
                 }
             }
             const uint64_t f = __ballot(pop != 0);
@@ -904,40 +923,57 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
 
 // Block-wide exclusive scan of gcnt[0, ng) in place; returns the total and the
 // sum of gstates[0, ng) to thread 0 (one workgroup of kScanThreads threads).
+// Tiles of kScanThreads * kScanPer elements: each thread loads kScanPer
+// consecutive counts (independent loads), the tile is scanned with wave
+// shuffles and one pass over the wave totals.
+constexpr int kScanPer = 8;
 
 __device__ void scan_groups(uint32_t* gcnt, const unsigned long long* gstates, uint32_t ng,
                             uint32_t& total, unsigned long long& states) {
-    __shared__ uint32_t part[kScanThreads];
-    __shared__ unsigned long long spart[kScanThreads / 64];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (ng + kScanThreads - 1) / kScanThreads;
-    const uint32_t lo = min(ng, t * per), hi = min(ng, lo + per);
-    uint32_t sum = 0;
+    constexpr int kWaves = kScanThreads / 64;
+    __shared__ uint32_t wsum[kWaves];
+    __shared__ unsigned long long wst[kWaves];
+    const uint32_t t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    uint32_t run = 0;
     unsigned long long st = 0;
-    for (uint32_t i = lo; i < hi; ++i) {
-        sum += gcnt[i];
-        st += gstates[i];
+    for (uint32_t base = 0; base < ng; base += kScanThreads * kScanPer) {
+        const uint32_t i0 = base + t * kScanPer;
+        uint32_t v[kScanPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int e = 0; e < kScanPer; ++e) {
+            v[e] = i0 + e < ng ? gcnt[i0 + e] : 0u;
+            st += i0 + e < ng ? gstates[i0 + e] : 0ull;
+            sum += v[e];
+        }
+        uint32_t incl = sum;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(incl, off);
+            if (lane >= static_cast<uint32_t>(off)) incl += u;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        uint32_t pre = 0, tile = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            if (static_cast<uint32_t>(w) < wid) pre += wsum[w];
+            tile += wsum[w];
+        }
+        uint32_t r = run + pre + incl - sum;
+#pragma unroll
+        for (int e = 0; e < kScanPer; ++e) {
+            if (i0 + e < ng) gcnt[i0 + e] = r;
+            r += v[e];
+        }
+        run += tile;
+        __syncthreads();  // wsum is reused by the next tile
     }
-    part[t] = sum;
     st = wave_sum(st);
-    if ((t & 63) == 0) spart[t >> 6] = st;
+    if (lane == 0) wst[wid] = st;
     __syncthreads();
-    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {  // Hillis-Steele inclusive scan
-        const uint32_t v = t >= off ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[t] - sum;  // exclusive prefix of this thread's chunk
-    for (uint32_t i = lo; i < hi; ++i) {
-        const uint32_t c = gcnt[i];
-        gcnt[i] = run;
-        run += c;
-    }
     if (t == 0) {
-        total = part[kScanThreads - 1];
+        total = run;
         states = 0;
-        for (int i = 0; i < kScanThreads / 64; ++i) states += spart[i];
+        for (int w = 0; w < kWaves; ++w) states += wst[w];
     }
     __syncthreads();
 }
@@ -1489,6 +1525,7 @@ struct cr_solver {
     uint64_t* d_log_bits = nullptr;
     uint64_t log_cap = 0;          // entries; 0: the lists write their depth bytes directly
     uint32_t log_factor = 24;      // tuning: log capacity in units of C entries (0: no log)
+    uint64_t log_min_configs = 1ull << 24;  // tuning: no log for fewer configurations
     LogList* d_lists = nullptr;    // descriptors of the depths k_apply_log applies
     uint32_t* d_starts = nullptr;  // k_log_chunks: per chunk and list, first entry and count
     LogList* h_lists = nullptr;    // pinned staging of the descriptors
@@ -1998,14 +2035,15 @@ int alloc_pull(cr_solver* s) {
 }
 
 // Depth log: up to log_factor * C entries of (configuration, W robber
-// words), within half of the free device memory (none below 2 * C entries).
-// A depth whose lists do not fit in the remaining room writes its bytes
-// directly.
+// words), within half of the free device memory (none below 2 * C entries,
+// and none for fewer than log_min_configs configurations: small tables take
+// their few bytes per depth directly). A depth whose lists do not fit in the
+// remaining room writes its bytes directly (k_log_lists).
 constexpr int kMaxLogLists = 256;
 uint32_t log_chunk_rows(const cr_solver* s);
 
 int alloc_log(cr_solver* s) {
-    if (!s->log_factor) return 0;
+    if (!s->log_factor || s->C < s->log_min_configs) return 0;
     const uint64_t entry = 4 + 8ull * s->W;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
@@ -2414,7 +2452,7 @@ int do_loop(cr_solver* s) {
             t->stop();
             hipLaunchKernelGGL(k_advance, dim3(1), dim3(kScanThreads), 0, s->stream, a);
             hipLaunchKernelGGL(k_write_lists, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, a, 0);
-            if ((rc = dispatch_w<LaunchLogLists>(s->W, s, a)) != 0) return rc;
+            if (s->log_cap && (rc = dispatch_w<LaunchLogLists>(s->W, s, a)) != 0) return rc;
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl, offsetof(Ctl, lvl), hipMemcpyDeviceToHost, s->stream));
@@ -2924,6 +2962,11 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
         case CR_TUNE_DEPTH_LOG:
             if (value < 0 || value > 1024) return fail(CR_ERR_ARG, "depth log %lld", static_cast<long long>(value));
             s->log_factor = static_cast<uint32_t>(value);
+            if (s->stage > kLoaded) s->stage = kLoaded;
+            return 0;
+        case CR_TUNE_DEPTH_LOG_MIN:
+            if (value < 0) return fail(CR_ERR_ARG, "depth log minimum %lld", static_cast<long long>(value));
+            s->log_min_configs = static_cast<uint64_t>(value);
             if (s->stage > kLoaded) s->stage = kLoaded;
             return 0;
         case CR_TUNE_PULL_TILE_BLOCKS:
