@@ -118,6 +118,8 @@ struct LevelArgs {
     unsigned long long* gcaps;  // per group of k_init: capture states
     Ctl* ctl;
     const uint64_t* out_mask;  // [n][4] robber moves
+    const uint64_t* in_mask;   // [n][4] robber positions with a move onto each vertex
+    uint64_t* fr0;             // per configuration: the cop-turn states won at the latest depth
     uint32_t nbatch;           // ceil(C / 64)
     int32_t filter;
     uint32_t pull_rows;        // a level pulls when its robber frontier has more rows than this
@@ -161,8 +163,21 @@ __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint64_t lds_om[CR_MAX_N * W];
     __shared__ uint64_t stage[kWavesPerBlock][3][W][64];  // per wave: its rows' cm, c1, r1 words
+    __shared__ uint64_t lds_im[CR_MAX_N * W];
+    __shared__ uint64_t lnm[W];  // robbers without any move
     load_image(lds, gv.image, gv.image_bytes);
     const uint64_t* lom = stage_out_masks<W>(lds_om, a.out_mask, gv.n);
+    const uint64_t* lim = stage_out_masks<W>(lds_im, a.in_mask, gv.n);
+    if (threadIdx.x < W) {
+        uint64_t m = 0;
+        for (int r = 64 * threadIdx.x; r < min(gv.n, 64 * static_cast<int>(threadIdx.x) + 64); ++r) {
+            bool none = true;
+            for (int q = 0; q < W; ++q) none = none && lom[r * W + q] == 0;
+            if (none) m |= 1ull << (r & 63);
+        }
+        lnm[threadIdx.x] = m;
+    }
+    __syncthreads();
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const int lane = threadIdx.x & 63;
     const uint32_t wid = wave_id();
@@ -178,8 +193,8 @@ __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
             uint64_t cm[W], am[W], r1[W];
 #pragma unroll
             for (int q = 0; q < W; ++q) cm[q] = am[q] = r1[q] = 0;
+            uint32_t cops[K];
             if (live) {
-                uint32_t cops[K];
                 unrank<K>(row, gv, binom, cops);
 #pragma unroll
                 for (int i = 0; i < K; ++i) {
@@ -190,13 +205,27 @@ __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
                     }
                 }
             }
-            // robber vertices in a uniform loop (out-masks broadcast from LDS)
+            // a robber-turn state is won at depth 1 iff every move of r lands on a
+            // cop: r has a move onto some cop (candidates from the transposed
+            // moves) and all its moves are onto cops; robbers without any move
+            // have none onto a cop and are tested in a uniform pass
+            uint64_t cand[W];
+#pragma unroll
+            for (int q = 0; q < W; ++q) cand[q] = lnm[q];
+            if (live) {
+#pragma unroll
+                for (int i = 0; i < K; ++i)
+#pragma unroll
+                    for (int q = 0; q < W; ++q) cand[q] |= lim[cops[i] * W + q];
+            }
 #pragma unroll
             for (int w = 0; w < W; ++w) {
-                const int rend = min(64, n - 64 * w);
-                for (int rb = 0; rb < rend; ++rb) {
+                uint64_t x = cand[w] & ~cm[w];
+                while (x) {
+                    const int rb = __builtin_ctzll(x);
+                    x &= x - 1;
                     const uint64_t* om = lom + (64 * w + rb) * W;
-                    uint64_t bad = (cm[w] >> rb) & 1;
+                    uint64_t bad = 0;
 #pragma unroll
                     for (int q = 0; q < W; ++q) bad |= om[q] & ~cm[q];
                     r1[w] |= static_cast<uint64_t>(bad == 0) << rb;
@@ -222,6 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
                     a.won0[o + q] = cm[q] | c1[q];
                     a.won1[o + q] = cm[q] | r1[q];
                     a.fr1[o + q] = r1[q];
+                    a.fr0[o + q] = c1[q];
                     a.nx0[o + q] = 0;
                     lc += __popcll(cm[q]);
                     l0 += __popcll(c1[q]);
@@ -791,9 +821,11 @@ __global__ __launch_bounds__(kBlock) void k_clear_frontier(LevelArgs a, int32_t 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs a) {
     __shared__ uint64_t lds_om[CR_MAX_N * W];
+    __shared__ uint64_t lds_im[CR_MAX_N * W];
     const int lane = threadIdx.x & 63;
     if (a.ctl->done) return;
     const uint64_t* lom = stage_out_masks<W>(lds_om, a.out_mask, gv.n);
+    const uint64_t* lim = stage_out_masks<W>(lds_im, a.in_mask, gv.n);
     const uint32_t level = a.ctl->level, cur = level & 1;
     const uint32_t count = a.ctl->cnt0[cur];
     const uint32_t* list = a.list0[cur];
@@ -811,19 +843,35 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
             const bool live = i < count;
             const uint32_t row = live ? list[i] : 0u;
             const uint64_t o = static_cast<uint64_t>(row) * W;
-            uint64_t w0[W], w1[W], nw[W];
+            uint64_t w0[W], w1[W], f0[W], nw[W], cand[W];
 #pragma unroll
             for (int q = 0; q < W; ++q) {
                 w0[q] = live ? a.won0[o + q] : 0ull;
                 w1[q] = live ? a.won1[o + q] : ~0ull;
+                f0[q] = live ? a.fr0[o + q] : 0ull;
                 nw[q] = 0;
+                cand[q] = 0;
+            }
+            // Only a robber with a move onto a cop-turn state won at this depth
+            // can change: its other moves were already counted at earlier depths.
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                uint64_t x = f0[w];
+                while (x) {
+                    const uint64_t* im = lim + (64 * w + __builtin_ctzll(x)) * W;
+                    x &= x - 1;
+#pragma unroll
+                    for (int q = 0; q < W; ++q) cand[q] |= im[q];
+                }
             }
 #pragma unroll
             for (int w = 0; w < W; ++w) {
-                const int rend = min(64, n - 64 * w);
-                for (int rb = 0; rb < rend; ++rb) {
+                uint64_t x = cand[w] & ~w1[w];
+                while (x) {
+                    const int rb = __builtin_ctzll(x);
+                    x &= x - 1;
                     const uint64_t* om = lom + (64 * w + rb) * W;
-                    uint64_t bad = (w1[w] >> rb) & 1;
+                    uint64_t bad = 0;
 #pragma unroll
                     for (int q = 0; q < W; ++q) bad |= om[q] & ~w0[q];
                     nw[w] |= static_cast<uint64_t>(bad == 0) << rb;
@@ -887,24 +935,26 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
             }
             uint32_t pop = 0;
             if (any) {  // this lane's row has candidates: settle them
+                uint64_t nw[W];
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
                     const uint64_t wv = a.won0[o + q];
-                    const uint64_t nw = x[q] & ~wv;
-                    pop += __popcll(nw);
-                    if (nw) a.won0[o + q] = wv | nw;
-                    if (a.log_rows) {
-                        // the new states stay in nx0 until k_log_lists logs them and
-                        // clears the row
-                        a.nx0[o + q] = nw;
-                        continue;
-                    }
+                    nw[q] = x[q] & ~wv;
+                    pop += __popcll(nw[q]);
+                    if (nw[q]) a.won0[o + q] = wv | nw[q];
                     a.nx0[o + q] = 0;
-                    uint64_t bits = nw;
-                    while (bits) {
-                        const int r = q * 64 + __builtin_ctzll(bits);
-                        bits &= bits - 1;
-                        a.table[(static_cast<uint64_t>(row) * gv.n + r) * 2] = nb;
+                }
+                if (pop) {  // a listed row: its new states for the robber step and k_log_lists
+#pragma unroll
+                    for (int q = 0; q < W; ++q) {
+                        a.fr0[o + q] = nw[q];
+                        if (a.log_rows) continue;  // the depth bytes go through the log
+                        uint64_t bits = nw[q];
+                        while (bits) {
+                            const int r = q * 64 + __builtin_ctzll(bits);
+                            bits &= bits - 1;
+                            a.table[(static_cast<uint64_t>(row) * gv.n + r) * 2] = nb;
+                        }
                     }
                 }
             }
@@ -1095,9 +1145,8 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
 
 // The depth bytes of the states listed by k_write_lists (not in init mode):
 // one thread per list entry takes the row's new robber bits (fr1 for the
-// robber turn; nx0 for the cop turn, left there by k_cop_finalize and cleared
-// here) into the depth log, or, when the log has no room for this depth,
-// writes the bytes into the table directly.
+// robber turn, fr0 for the cop turn) into the depth log, or, when the log has
+// no room for this depth, writes the bytes into the table directly.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
     const Ctl* ctl = a.ctl;
@@ -1113,15 +1162,9 @@ __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
         const bool robber = i >= n0;
         const uint32_t k = static_cast<uint32_t>(robber ? i - n0 : i);
         const uint32_t row = (robber ? a.list1[nxt] : a.list0[nxt])[k];
-        uint64_t* src = (robber ? a.fr1 : a.nx0) + static_cast<uint64_t>(row) * W;
+        const uint64_t* src = (robber ? a.fr1 : a.fr0) + static_cast<uint64_t>(row) * W;
         uint64_t w[W];
         load_row<W>(src, w);
-        if (!robber) {
-            uint64_t z[W];
-#pragma unroll
-            for (int q = 0; q < W; ++q) z[q] = 0;
-            store_row<W>(src, z);
-        }
         if (logging) {
             const unsigned long long e = (robber ? off1 : off0) + k;
             a.log_rows[e] = row;
@@ -1518,6 +1561,7 @@ struct cr_solver {
     uint64_t* d_won0 = nullptr;
     uint64_t* d_won1 = nullptr;
     uint64_t* d_fr1 = nullptr;
+    uint64_t* d_fr0 = nullptr;    // per configuration: the cop-turn states its last depth added
     uint64_t* d_nx0 = nullptr;
     uint32_t* d_list[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [turn][parity] dense row lists
     // depth log (k_write_lists, k_apply_log)
@@ -1541,6 +1585,7 @@ struct cr_solver {
     Ctl* d_ctl = nullptr;
     uint8_t* d_image = nullptr;
     uint64_t* d_outmask = nullptr;
+    uint64_t* d_inmask = nullptr;  // [n][4]: robber positions with a move onto each vertex
     Ctl* h_ctl = nullptr;  // pinned
     uint64_t device_bytes = 0;
     GraphView gv{};
@@ -1630,6 +1675,10 @@ struct cr_solver {
         hipFree(d_ctl);
         hipFree(d_image);
         hipFree(d_outmask);
+        hipFree(d_inmask);
+        hipFree(d_fr0);
+        d_inmask = nullptr;
+        d_fr0 = nullptr;
         d_table = nullptr;
         d_won0 = d_won1 = d_fr1 = d_nx0 = nullptr;
         d_ctl = nullptr;
@@ -1906,6 +1955,8 @@ LevelArgs level_args(cr_solver* s) {
     a.pull_rows = s->pull_rows;
     a.ctl = s->d_ctl;
     a.out_mask = s->d_outmask;
+    a.in_mask = s->d_inmask;
+    a.fr0 = s->d_fr0;
     a.filter = s->filter;
     return a;
 }
@@ -1956,6 +2007,13 @@ int upload_graph(cr_solver* s) {
     s->d_outmask = dev_alloc<uint64_t>(om.size(), s->device_bytes, rc);
     if (!s->d_outmask) return rc;
     HIP_TRY(hipMemcpy(s->d_outmask, om.data(), om.size() * 8, hipMemcpyHostToDevice));
+    std::vector<uint64_t> im(om.size(), 0);  // transpose of the robber moves
+    for (int r = 0; r < n; ++r)
+        for (int v = 0; v < n; ++v)
+            if ((om[r * 4 + v / 64] >> (v % 64)) & 1) im[v * 4 + r / 64] |= 1ull << (r % 64);
+    s->d_inmask = dev_alloc<uint64_t>(im.size(), s->device_bytes, rc);
+    if (!s->d_inmask) return rc;
+    HIP_TRY(hipMemcpy(s->d_inmask, im.data(), im.size() * 8, hipMemcpyHostToDevice));
     s->gv.image = s->d_image;
     s->gv.image_bytes = static_cast<uint32_t>(img.size());
     s->gv.off_binom = off_binom;
@@ -2228,7 +2286,7 @@ int do_build(cr_solver* s) {
     s->S = s->C * static_cast<uint64_t>(n) * s->columns;
     s->W = (n + 63) / 64;
     const uint64_t plane = s->C * s->W;
-    const uint64_t need = s->S + 4 * plane * 8 + 4 * s->C * 4 + 2 * s->C / 4 + (4 << 20);
+    const uint64_t need = s->S + 5 * plane * 8 + 4 * s->C * 4 + 2 * s->C / 4 + (4 << 20);
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     if (need > freeb) return fail(CR_ERR_TOO_LARGE, "state space needs %llu bytes; %zu free", static_cast<unsigned long long>(need), freeb);
@@ -2237,6 +2295,7 @@ int do_build(cr_solver* s) {
     if (!(s->d_won0 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
     if (!(s->d_won1 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
     if (!(s->d_fr1 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
+    if (!(s->d_fr0 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
     if (!(s->d_nx0 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
     for (auto& t : s->d_list)
         for (auto& b : t)
