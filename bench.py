@@ -58,7 +58,7 @@ MACRO_WORKLOADS = {
     "grid8x8_k2_p2": ("grid8x8_chords16_s1", 2, 2),
 }
 # solver timer name -> device kernel name in rocprofv3 summaries
-KERNEL_NAMES = {"cop_push": "k_cop_push", "sep_stage": "k_sep_stage", "robber_step": "k_robber_step",
+KERNEL_NAMES = {"cop_push": "k_cop_push", "sep_stage": "k_pull_tile", "robber_step": "k_robber_step",
                 "cop_finalize": "k_cop_finalize", "clear_frontier": "k_clear_frontier",
                 "mx_expand": "k_mx_expand", "mx_emit": "k_mx_expand", "mx_insert": "k_mx_insert",
                 "mx_compact": "k_mx_compact", "mx_leaf": "k_mx_leaf", "mx_up": "k_mx_up",

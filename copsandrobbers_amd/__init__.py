@@ -38,7 +38,7 @@ EXPORTS = (
     "cr_load_edge_file", "cr_build_aux_graph",
     "cr_initialize_captures", "cr_main_loop", "cr_find_final_result", "cr_save_aux_graph",
     "cr_get_info", "cr_get_stats", "cr_copy_states", "cr_copy_configs", "cr_level_counts",
-    "cr_count_retro_edges", "cr_work_profile", "cr_get_macro_stats", "cr_write_dp_table", "cr_extract_game",
+    "cr_count_retro_edges", "cr_work_profile", "cr_level_rows", "cr_get_macro_stats", "cr_write_dp_table", "cr_extract_game",
     "cr_write_perfect_game", "cr_cop_number", "cr_count_transitions",
     "cr_set_tuning", "cr_set_kernel_timing", "cr_set_kernel_timing_filter", "cr_kernel_times", "cr_cache_key",
     "cr_batch_create", "cr_batch_destroy", "cr_batch_solve", "cr_batch_set_tuning", "cr_batch_pass_counts",
@@ -136,6 +136,7 @@ def lib():
         L.cr_count_retro_edges.argtypes = [vp, u64p, u64p, u64p]
         L.cr_count_transitions.argtypes = [vp, u64p]
         L.cr_work_profile.argtypes = [vp, u64p, u64p, u8p]
+        L.cr_level_rows.argtypes = [vp, u64p]
         L.cr_get_macro_stats.argtypes = [vp, ctypes.POINTER(MacroStats), u64p, u64p, i32]
         L.cr_set_tuning.argtypes = [vp, i32, ctypes.c_int64]
         L.cr_set_kernel_timing.argtypes = [vp, i32]
@@ -284,6 +285,13 @@ class Solver:
         out = np.empty((count, info.k), np.uint8)
         _check(lib().cr_copy_configs(self._h, _u8(out), first, count))
         return out
+
+    def level_rows(self):
+        """Per depth: configurations with cop-turn / robber-turn states of that
+        depth (the level lists), shape (127, 2)."""
+        out = np.zeros(2 * (MAX_ROUND), np.uint64)
+        _check(lib().cr_level_rows(self._h, _u64(out)))
+        return out.reshape(MAX_ROUND, 2)
 
     def level_counts(self):
         out = np.zeros(2 * MAX_ROUND, np.uint64)

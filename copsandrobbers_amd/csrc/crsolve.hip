@@ -85,6 +85,7 @@ struct Ctl {
     unsigned long long log_off[128][2];  // per depth and turn: first entry
     uint32_t log_len[128][2];            // per depth and turn: entries (0 when written directly)
     unsigned long long lvl[128][2];
+    uint32_t rows[128][2];  // per depth: configurations listed with new cop-turn / robber-turn states
     unsigned long long hist[2][128];  // retro-edges by depth: [0] cop (robber states), [1] robber (cop states)
     unsigned long long verdict;
     unsigned long long retro[3];
@@ -1051,6 +1052,8 @@ __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
         ctl->lvl[d + 1][1] = s1;
         ctl->cnt0[nxt] = t0;
         ctl->cnt1[nxt] = t1;
+        ctl->rows[d + 1][0] = t0;
+        ctl->rows[d + 1][1] = t1;
         ctl->pull = choose_pull(t1, a.pull_rows);
         ctl->dir[d + 1] = ctl->pull;
         ctl->level = d + 1;
@@ -1094,6 +1097,8 @@ __global__ __launch_bounds__(kScanThreads) void k_settle_init(LevelArgs a) {
         ctl->lvl[1][1] = s1;
         ctl->cnt0[1] = t0;
         ctl->cnt1[1] = t1;
+        ctl->rows[1][0] = t0;
+        ctl->rows[1][1] = t1;
         ctl->pull = choose_pull(t1, a.pull_rows);
         ctl->dir[1] = ctl->pull;
         ctl->level = 1;
@@ -1289,30 +1294,49 @@ __global__ __launch_bounds__(kBlock) void k_apply_log(const LogList* lists, int3
         // R0 * 2n is a multiple of 16), then the tail bytes
         uint8_t* trow = table + static_cast<uint64_t>(R0) * rowbytes;
         const uint32_t nbytes = (R1 - R0) * rowbytes, nvec = nbytes / 16;
-        for (uint32_t i = threadIdx.x; i < nvec; i += blockDim.x)
-            reinterpret_cast<uint4*>(lds)[i] = reinterpret_cast<const uint4*>(trow)[i];
+        for (uint32_t i0 = threadIdx.x; i0 < nvec; i0 += blockDim.x * 4) {  // four loads in flight
+            uint4 x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i0 + u * blockDim.x < nvec) x[u] = reinterpret_cast<const uint4*>(trow)[i0 + u * blockDim.x];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i0 + u * blockDim.x < nvec) reinterpret_cast<uint4*>(lds)[i0 + u * blockDim.x] = x[u];
+        }
         for (uint32_t i = nvec * 16 + threadIdx.x; i < nbytes; i += blockDim.x) lds[i] = trow[i];
         __syncthreads();
-        for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-            int lo = 0, hi = nl - 1;  // the list holding the chunk's entry e (the last with prefix <= e)
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_pre[mid] <= e) lo = mid; else hi = mid - 1;
-            }
-            const unsigned long long ent = s_off[lo] + s_beg[lo] + (e - s_pre[lo]);
-            const uint32_t row = rows[ent];
-            uint64_t w[W];
-            load_row<W>(bits + ent * W, w);
-            const uint32_t tb = s_tb[lo];
-            uint8_t* rp = lds + (row - R0) * rowbytes + (tb >> 8);
-            const uint8_t byte = static_cast<uint8_t>(tb & 0xFF);
+        constexpr int kU = 4;  // entries per thread whose loads are in flight together
+        for (uint32_t e0 = threadIdx.x; e0 < total; e0 += blockDim.x * kU) {
+            uint32_t row[kU], tb[kU];
+            uint64_t w[kU][W];
 #pragma unroll
-            for (int q = 0; q < W; ++q) {
-                uint64_t x = w[q];
-                while (x) {
-                    const uint32_t r = q * 64 + __builtin_ctzll(x);
-                    x &= x - 1;
-                    if (r < n) rp[2 * r] = byte;
+            for (int u = 0; u < kU; ++u) {
+                const uint32_t e = e0 + u * blockDim.x;
+                tb[u] = 0xFFFFFFFFu;
+                if (e >= total) continue;
+                int lo = 0, hi = nl - 1;  // the list holding the chunk's entry e (the last with prefix <= e)
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= e) lo = mid; else hi = mid - 1;
+                }
+                const unsigned long long ent = s_off[lo] + s_beg[lo] + (e - s_pre[lo]);
+                row[u] = rows[ent];
+                load_row<W>(bits + ent * W, w[u]);
+                tb[u] = s_tb[lo];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (tb[u] == 0xFFFFFFFFu) continue;
+                uint8_t* rp = lds + (row[u] - R0) * rowbytes + (tb[u] >> 8);
+                const uint8_t byte = static_cast<uint8_t>(tb[u] & 0xFF);
+#pragma unroll
+                for (int q = 0; q < W; ++q) {
+                    uint64_t x = w[u][q];
+                    while (x) {
+                        const uint32_t r = q * 64 + __builtin_ctzll(x);
+                        x &= x - 1;
+                        if (r < n) rp[2 * r] = byte;
+                    }
                 }
             }
         }
@@ -1386,30 +1410,45 @@ __global__ __launch_bounds__(kBlock) void k_verdict(GraphView gv, LevelArgs a, i
         const int bits = min(64, n - 64 * q);
         full[q] = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
     }
-    const bool vec = columns == 2 && (n & 7) == 0;  // rows of 2n bytes, 16-byte aligned
+    // two-column rows of 2n bytes, 16-byte aligned: a wave's 64 rows are one
+    // contiguous span, read in consecutive 16-byte chunks with the per-row
+    // maxima gathered in LDS
+    const bool vec = columns == 2 && (n & 7) == 0;
+    __shared__ uint32_t rmax[kWavesPerBlock][64];
+    const uint32_t wid = threadIdx.x >> 6;
     unsigned long long best = ~0ull;
     const uint32_t nbatch = (gv.C + 63) / 64;
-    for (uint32_t b = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); b < nbatch;
-         b += gridDim.x * kWavesPerBlock) {
+    for (uint32_t b = blockIdx.x * kWavesPerBlock + wid; b < nbatch; b += gridDim.x * kWavesPerBlock) {
         const uint32_t row = b * 64 + lane;
         bool uni = row < gv.C;
 #pragma unroll
         for (int q = 0; q < W; ++q) uni = uni && a.won0[static_cast<uint64_t>(row) * W + q] == full[q];
-        if (!uni) continue;
-        const uint8_t* t = a.table + static_cast<uint64_t>(row) * n * columns;
-        uint32_t top = 0;  // largest cop-turn byte of the row
+        if (__ballot(uni) == 0) continue;  // uniform per wave
+        uint32_t top = 0;  // largest cop-turn byte of the lane's row
         if (vec) {
-            const uint4* v = reinterpret_cast<const uint4*>(t);
-            for (int i = 0; i < n / 8; ++i) {
+            rmax[wid][lane] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t rows_here = min(64u, gv.C - b * 64), cpr = static_cast<uint32_t>(n) / 8;
+            const uint4* v = reinterpret_cast<const uint4*>(a.table + static_cast<uint64_t>(b) * 64 * n * 2);
+            for (uint32_t i = lane; i < rows_here * cpr; i += 64) {
                 const uint4 x = v[i];
                 const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+                uint32_t m = 0;
 #pragma unroll
-                for (int h = 0; h < 4; ++h) top = max(top, max(w[h] & 0xFFu, (w[h] >> 16) & 0xFFu));
+                for (int h = 0; h < 4; ++h) m = max(m, max(w[h] & 0xFFu, (w[h] >> 16) & 0xFFu));
+                atomicMax(&rmax[wid][i / cpr], m);
             }
-        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            top = rmax[wid][lane];
+        } else if (uni) {
+            const uint8_t* t = a.table + static_cast<uint64_t>(row) * n * columns;
             for (int r = 0; r < n; ++r) top = max(top, static_cast<uint32_t>(t[static_cast<uint64_t>(r) * columns]));
         }
-        best = min(best, (static_cast<unsigned long long>(top >> 1) << 32) | row);
+        if (uni) best = min(best, (static_cast<unsigned long long>(top >> 1) << 32) | row);
     }
     for (int off = 32; off > 0; off >>= 1) best = min(best, static_cast<unsigned long long>(__shfl_xor(best, off)));
     if (lane == 0 && best != ~0ull) atomicMin(&a.ctl->verdict, best);
@@ -1592,7 +1631,7 @@ struct cr_solver {
     // tuning knobs (cr_set_tuning)
     int32_t filter = 1;
     int32_t use_csr = 0;  // precompute predecessor lists (off by default: no gain measured)
-    uint64_t pull_scale = 500;    // pull threshold scale (permille of the estimated break-even)
+    int64_t pull_scale = -1;      // pull threshold scale (permille of the estimated break-even); -1: by size
     uint64_t* d_sep[2] = {nullptr, nullptr};  // factorised-pull intermediates H_i
     uint64_t ms_count[CR_MAX_K + 1] = {};     // C(n+j-1, j): multisets of size j
     bool pull_tiles = true;                   // k_pull_tile (else k_sep_stage) runs the pull stages
@@ -2064,7 +2103,12 @@ void set_pull_threshold(cr_solver* s) {
     double pull_reads = 0;
     for (int i = 1; i <= K; ++i) pull_reads += static_cast<double>(s->ms_count[i]) * s->ms_count[K - i] * deg;
     const double pred = std::pow(std::max(1.0, static_cast<double>(edges) / n), K);
-    const double rows = pull_reads / pred * static_cast<double>(s->pull_scale) / 1000.0;
+    // Measured on MI355X: a push costs ~4-6 ns per frontier row while the
+    // won0/nx0 bitsets fit in the 256 MiB last-level cache and ~10 ns beyond,
+    // a pull ~0.007 ns per row read; the default scale follows that split.
+    const bool cached = 2ull * s->C * s->W * 8 <= (256ull << 20);
+    const int64_t scale = s->pull_scale >= 0 ? s->pull_scale : (cached ? 500 : 190);
+    const double rows = pull_reads / pred * static_cast<double>(scale) / 1000.0;
     s->pull_rows = rows >= 4.0e9 ? 0xffffffffu : static_cast<uint32_t>(rows);
 }
 
@@ -2887,6 +2931,19 @@ int cr_work_profile(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint8_t* pul
     return 0;
 }
 
+int cr_level_rows(cr_solver* s, uint64_t* rows) {
+    if (int rc = check_solver(s)) return rc;
+    if (rows == nullptr) return fail(CR_ERR_ARG, "null output");
+    if (s->variant == CR_VARIANT_MACRO || s->p != 1)
+        return fail(CR_ERR_UNSUPPORTED, "depth levels are kept for the p = 1 column solver");
+    if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    for (int d = 0; d < 127; ++d) {
+        rows[2 * d] = s->h_ctl->rows[d][0];
+        rows[2 * d + 1] = s->h_ctl->rows[d][1];
+    }
+    return 0;
+}
+
 int cr_get_macro_stats(cr_solver* s, cr_macro_stats* out, uint64_t* nodes_per_step, uint64_t* edges_per_step,
                        int32_t cap) {
     if (int rc = check_solver(s)) return rc;
@@ -3013,8 +3070,8 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             s->row_blocks = static_cast<uint32_t>(value);
             return 0;
         case CR_TUNE_PULL_SCALE:
-            if (value < 0) return fail(CR_ERR_ARG, "pull scale %lld", static_cast<long long>(value));
-            s->pull_scale = static_cast<uint64_t>(value);
+            if (value < 0) value = -1;  // the size-dependent default
+            s->pull_scale = value;
             set_pull_threshold(s);
             return 0;
         case CR_TUNE_PULL_TILES: s->pull_tiles = value != 0; return 0;

@@ -163,6 +163,10 @@ int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_
  * pushing (depth-0 relaxations are resolved in closed form by
  * cr_initialize_captures). */
 int cr_work_profile(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint8_t* pulled);
+/* Per depth d (127 entries of two): the configurations with cop-turn
+ * ([2d]) and robber-turn ([2d + 1]) states of depth d -- the lengths of the
+ * level lists. No counterpart in the reference (it has no level lists). */
+int cr_level_rows(cr_solver* s, uint64_t* rows);
 /* Limited-visibility variant (CR_VARIANT_MACRO): the trajectory DAG of the last
  * mainLoop and the work it did. No counterpart in the reference, whose
  * trajectory search (k_cops_visibility_v2.cpp:243-341) keeps no such graph. */
@@ -226,7 +230,8 @@ enum cr_tuning {
     CR_TUNE_PRED_CSR = 5,        /* 1: cr_build_aux_graph precomputes predecessor lists (default 0) */
     CR_TUNE_PULL_SCALE = 6,      /* a level gathers (factorised pull) instead of pushing when its robber
                                     frontier exceeds value/1000 times the estimated break-even row count;
-                                    0: always pull, >= 10^9: never (default 500) */
+                                    0: always pull, >= 10^9: never; default 500 while the cop-turn
+                                    bitsets fit in 256 MiB, else 190; a negative value restores it */
     CR_TUNE_PULL_TILES = 7,      /* 1 (default): the pull's stages load their input rows through LDS
                                     tiles; 0: one row per lane with direct gathers */
     CR_TUNE_PULL_TILE_ROWS = 8,  /* rows of each input block per LDS tile, 1..16 (default 2) */

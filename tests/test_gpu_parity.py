@@ -93,6 +93,13 @@ def test_levels_and_retro_edges_match_oracle(solver, oracle):
         states, levels, e_cop, e_rob = oracle.v1_levelsync(adj, k, nthreads=8)
         assert np.array_equal(solver.states(), states)
         assert np.array_equal(solver.level_counts(), levels)
+        # configurations holding states of each depth (the level lists)
+        st = states.reshape(-1, n, 2)
+        dep = np.where(st & 1, st >> 1, 127)
+        rows = solver.level_rows()
+        for d in range(1, 40):
+            for t in range(2):
+                assert rows[d, t] == int((dep[:, :, t] == d).any(axis=1).sum()), (graph, d, t)
         got = solver.retro_edges()
         assert got == (e_cop, e_rob, int((states & 1).sum()))
 
@@ -204,7 +211,7 @@ PULL_SETTINGS = [{"pull_scale": 0, "pull_tiles": 1, "pull_tile_rows": 2},
                  {"depth_log": 24, "depth_log_min": 0},
                  {"depth_log": 2, "depth_log_min": 0, "levels_per_sync": 3}]
 PULL_IDS = ["pull_tiles2", "pull_tiles3", "pull_rows", "push_only", "direct_bytes", "depth_log", "small_log"]
-DEFAULTS = {"pull_scale": 500, "pull_tiles": 1, "pull_tile_rows": 2, "depth_log": 24, "depth_log_min": 1 << 24,
+DEFAULTS = {"pull_scale": -1, "pull_tiles": 1, "pull_tile_rows": 2, "depth_log": 24, "depth_log_min": 1 << 24,
             "levels_per_sync": 8}
 PULL_CASES = ["v1__cycle100__k2__p1", "v1__dodecahedron__k3__p1", "v1__grid36__k3__p1", "v1__grid9__k3__p1",
               "v1__line3__k2__p1", "v1__outerplanar8__k3__p1", "v1__peterson__k3__p1", "v1__planar32__k3__p1",
