@@ -1607,7 +1607,7 @@ struct cr_solver {
     uint32_t* d_log_rows = nullptr;
     uint64_t* d_log_bits = nullptr;
     uint64_t log_cap = 0;          // entries; 0: the lists write their depth bytes directly
-    uint32_t log_factor = 24;      // tuning: log capacity in units of C entries (0: no log)
+    uint32_t log_factor = 40;      // tuning: log capacity in units of C entries (0: no log)
     uint64_t log_min_configs = 1ull << 24;  // tuning: no log for fewer configurations
     LogList* d_lists = nullptr;    // descriptors of the depths k_apply_log applies
     uint32_t* d_starts = nullptr;  // k_log_chunks: per chunk and list, first entry and count
@@ -2163,9 +2163,9 @@ int alloc_log(cr_solver* s) {
     return 0;
 }
 
-// Rows per chunk of k_apply_log (even, 48 KiB of LDS).
+// Rows per chunk of k_apply_log (a multiple of 8, 24 KiB of LDS).
 uint32_t log_chunk_rows(const cr_solver* s) {
-    return std::max<uint32_t>(8, std::min<uint32_t>(8192, 48 * 1024 / (2 * s->g.n))) & ~7u;
+    return std::max<uint32_t>(8, std::min<uint32_t>(8192, 24 * 1024 / (2 * s->g.n))) & ~7u;
 }
 
 template <int W>
@@ -2182,7 +2182,7 @@ struct LaunchApplyLog {
         const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((total + kBlock - 1) / kBlock, 8192)));
         hipLaunchKernelGGL(k_log_chunks, dim3(sgrid), dim3(kBlock), 0, s->stream, s->d_lists, nl, total, s->d_log_rows,
                            beg, cnt, R);
-        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nchunks, 4096));
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nchunks, 8192));
         if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_apply_log<W>), lds)) return rc;
         hipLaunchKernelGGL(k_apply_log<W>, dim3(grid), dim3(kBlock), lds, s->stream, s->d_lists, nl, s->d_log_rows,
                            s->d_log_bits, beg, cnt, s->d_table, n, static_cast<uint32_t>(s->C), R);

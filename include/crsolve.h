@@ -237,7 +237,7 @@ enum cr_tuning {
     CR_TUNE_PULL_TILE_ROWS = 8,  /* rows of each input block per LDS tile, 1..16 (default 2) */
     CR_TUNE_PULL_TILE_BLOCKS = 9, /* workgroups of a tiled pull stage (default 8192) */
     CR_TUNE_DEPTH_LOG = 10,      /* capacity of the depth log in units of the configuration count
-                                    (default 24; 0: no log, every depth writes its bytes directly):
+                                    (default 40; 0: no log, every depth writes its bytes directly):
                                     each depth's new states are logged and written to the table in row
                                     chunks. Takes effect at the next cr_build_aux_graph */
     CR_TUNE_DEPTH_LOG_MIN = 11   /* configurations below which no depth log is kept (default 2^24) */
