@@ -625,10 +625,11 @@ __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a,
 // extends (at most J + 1 times).
 // A workgroup walks a contiguous chunk of tiles: B, A' and the output row
 // advance incrementally (one unrank per chunk), and the loads of the next
-// tile are held in registers while the current tile is computed from LDS
-// (two tile buffers after the graph image, n * r * W words each).
+// tile are held in registers while the current tile is computed from its
+// LDS buffer (n * r * W words after the graph image): the bytes in flight per
+// CU are bounded by registers, not by a second LDS buffer.
 constexpr int kTileMaxRows = 16;
-constexpr int kTileLoads = 8;  // input words a thread stages per tile: n * r * W <= kTileLoads * kBlock
+constexpr int kTileLoads = 16;  // input words a thread stages per tile: n * r * W <= kTileLoads * kBlock
 constexpr uint32_t kLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
 
 template <int I, int J, int W>
@@ -643,9 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a,
     const uint32_t n = gv.n;
     const uint32_t Mp = gv.M + 1;
     const uint32_t r = st.r, rW = r * W, words = n * rW;
-    uint64_t* buf[2];
-    buf[0] = reinterpret_cast<uint64_t*>(lds + ((gv.image_bytes + 15) & ~15u));
-    buf[1] = buf[0] + words;
+    uint64_t* buf = reinterpret_cast<uint64_t*>(lds + ((gv.image_bytes + 15) & ~15u));
     uint64_t full[W];
 #pragma unroll
     for (int q = 0; q < W; ++q) {
@@ -732,23 +731,23 @@ __global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a,
             }
         }
     };
-    auto store_tile = [&](int b) {
+    auto store_tile = [&]() {
 #pragma unroll
         for (int m = 0; m < kTileLoads; ++m) {
             const uint32_t i = threadIdx.x + m * kBlock;
-            if (i < words) buf[b][i] = stage_w[m];
+            if (i < words) buf[i] = stage_w[m];
         }
     };
 
     uint32_t rc = min(r, cnt_in - ra0);
     if (threadIdx.x == 0) tile_table(0, rc);
     load_tile(rc);
-    store_tile(0);
+    store_tile();
     __shared__ unsigned long long t_out[2];
     if (threadIdx.x == 0) t_out[0] = out_row;
     for (uint32_t k = 0; k < count; ++k) {
         const int cur = k & 1, nxt = cur ^ 1;
-        __syncthreads();  // tile k is in buf[cur]; tile k - 1 is finished with buf[nxt]
+        __syncthreads();  // tile k is in buf
         const uint32_t rc_cur = rc;
         const bool more = k + 1 < count;
         if (more) {  // advance to tile k + 1 and start its loads
@@ -773,7 +772,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a,
             }
             load_tile(rc);
         }
-        const uint64_t* tb = buf[cur];
+        const uint64_t* tb = buf;
         const uint32_t total = t_pre[cur][rc_cur];
         const uint64_t o0 = t_out[cur];
         for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) {
@@ -798,7 +797,10 @@ __global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a,
             }
             store_row<W>(st.out + o * W, acc);
         }
-        if (more) store_tile(nxt);
+        if (more) {  // every thread is done with tile k: the next one replaces it
+            __syncthreads();
+            store_tile();
+        }
     }
 }
 
@@ -1635,7 +1637,7 @@ struct cr_solver {
     uint64_t* d_sep[2] = {nullptr, nullptr};  // factorised-pull intermediates H_i
     uint64_t ms_count[CR_MAX_K + 1] = {};     // C(n+j-1, j): multisets of size j
     bool pull_tiles = true;                   // k_pull_tile (else k_sep_stage) runs the pull stages
-    uint32_t tile_rows = 2;                   // k_pull_tile: rows of each input block per tile
+    uint32_t tile_rows = 4;                   // k_pull_tile: rows of each input block per tile
     uint32_t tile_blocks = 8192;              // k_pull_tile: workgroups (each walks a chunk of tiles)
     uint32_t pull_rows = 0xffffffffu;         // frontier rows above which a level pulls
     uint32_t push_blocks = 2048;
@@ -1876,7 +1878,7 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
         st.r = I == 1 ? 1u : s->tile_rows;
         // LDS of a tile: image, base[n] and n * r rows; fewer rows per tile (or
         // the row kernel) where a large graph image leaves too little room
-        auto tile_lds = [&](uint32_t r) { return ((s->gv.image_bytes + 15) & ~15u) + 2 * s->g.n * r * W * 8; };
+        auto tile_lds = [&](uint32_t r) { return ((s->gv.image_bytes + 15) & ~15u) + s->g.n * r * W * 8; };
         auto fits = [&](uint32_t r) {
             return tile_lds(r) <= kLdsBytes && s->g.n * r * W <= static_cast<uint32_t>(kTileLoads * kBlock);
         };
