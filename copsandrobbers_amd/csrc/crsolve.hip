@@ -431,6 +431,7 @@ struct SepStage {
     uint64_t rows;        // rows of H_i
     int32_t stage;        // i, 1..K
     uint32_t r;           // k_pull_tile: rows of each input block per tile
+    uint32_t chunk;       // k_pull_tile: tiles a workgroup takes at a time
 };
 
 // Loads / stores of one W-word row: 16-byte vector accesses where the row is
@@ -629,13 +630,15 @@ __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a,
 // LDS buffer (n * r * W words after the graph image): the bytes in flight per
 // CU are bounded by registers, not by a second LDS buffer.
 constexpr int kTileMaxRows = 16;
-constexpr int kTileLoads = 16;  // input words a thread stages per tile: n * r * W <= kTileLoads * kBlock
+constexpr int kTileBlock = 256;  // threads of a tile workgroup
+constexpr int kTileLoads = 16;   // input words a thread stages per tile: n * r * W <= kTileLoads * kTileBlock
 constexpr uint32_t kLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
 
 template <int I, int J, int W>
-__global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a, SepStage st) {
+__global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArgs a, SepStage st) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint32_t t_lo[2][kTileMaxRows], t_pre[2][kTileMaxRows + 1];
+    __shared__ unsigned long long t_out[2];
     if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
     load_image(lds, gv.image, gv.image_bytes);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
@@ -653,24 +656,28 @@ __global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a,
     }
     const uint32_t cnt_in = st.cnt_a_in;        // rows per block: cnt(I - 1)
     const uint32_t ntr = (cnt_in + r - 1) / r;  // A'-ranges per B
-    const uint64_t ntiles = static_cast<uint64_t>(st.cnt_b) * ntr;
-    // tiles in 8 contiguous segments, one per XCD (the grid is a multiple of
-    // 8), each split into contiguous chunks, one per workgroup
-    const uint32_t x = blockIdx.x % kXcds, lb = blockIdx.x / kXcds, bpx = gridDim.x / kXcds;
-    const uint64_t s0 = ntiles * x / kXcds, s1 = ntiles * (x + 1) / kXcds;
-    const uint64_t my0 = s0 + (s1 - s0) * lb / bpx, my1 = s0 + (s1 - s0) * (lb + 1) / bpx;
-    if (my0 >= my1) return;  // uniform per workgroup
-    const uint32_t count = static_cast<uint32_t>(my1 - my0);
+    const uint32_t cnt_b = st.cnt_b;
+    const uint64_t ntiles = static_cast<uint64_t>(cnt_b) * ntr;
+    // Tiles are ordered A'-range major (tile = tr * cnt_b + rb) and handed out
+    // in chunks of st.chunk, chunk c to workgroup c mod gridDim: the resident
+    // workgroups work on neighbouring tiles, which share their A'-range, so
+    // the J + 1 loads of an input row fall close together in time and the
+    // repeats find it in the caches.
+    const uint64_t nchunks = (ntiles + st.chunk - 1) / st.chunk;
+    uint64_t ch = blockIdx.x;
+    if (ch >= nchunks) return;  // uniform per workgroup
 
-    // walk state (uniform): B with its rank terms, the tile's first A' row,
-    // and (thread 0) the A' themselves and the output row of the tile
+    // walk state (uniform): the tile, its B with rank terms and first A' row;
+    // thread 0 also keeps the A' of the range and the output row
     constexpr int J1 = J + 1;
     const uint32_t M1 = n + J;  // M of (J+1)-multisets
     const uint32_t rank0 = binom[J1 * Mp + M1] - 1;
-    uint32_t rb = static_cast<uint32_t>(my0 / ntr);
-    uint32_t ra0 = static_cast<uint32_t>(my0 - static_cast<uint64_t>(rb) * ntr) * r;
+    uint64_t tix = 0, ch_end = 0;
+    uint32_t rb = 0, ra0 = 0;
     uint32_t B[J > 0 ? J : 1], U[J > 0 ? J : 1], S[J > 0 ? J : 1];
-    if constexpr (J > 0) unrank_j(rb, J, n, binom, gv.M, B);
+    uint32_t Ap[I > 1 ? I - 1 : 1], Ap0[I > 1 ? I - 1 : 1];  // thread 0: A' walk, first A' of the range
+    unsigned long long base_tr = 0;  // thread 0: output row of (A'-range first A' + its max) in block 0
+    uint32_t last_total = 0;         // thread 0: outputs per tile of the current A'-range
     auto terms = [&]() {
 #pragma unroll
         for (int t = 0; t < J; ++t) {
@@ -678,23 +685,63 @@ __global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a,
             S[t] = binom[(J - t) * Mp + (M1 - 2 - B[t] - t)];
         }
     };
-    terms();
-    uint32_t Ap[I > 1 ? I - 1 : 1];
-    uint64_t out_row = 0;  // output row of the current tile's first output
-    if (threadIdx.x == 0) {
-        uint32_t Af[I];
-        if constexpr (I > 1) {
-            unrank_j(ra0, I - 1, n, binom, gv.M, Ap);
+    auto start_at = [&](uint64_t t) {  // walk state from scratch at tile t
+        tix = t;
+        ch_end = min(ntiles, (t / st.chunk + 1) * st.chunk);
+        const uint32_t tr = static_cast<uint32_t>(t / cnt_b);
+        rb = static_cast<uint32_t>(t - static_cast<uint64_t>(tr) * cnt_b);
+        ra0 = tr * r;
+        if constexpr (J > 0) unrank_j(rb, J, n, binom, gv.M, B);
+        terms();
+        if (threadIdx.x == 0) {
+            uint32_t Af[I];
+            if constexpr (I > 1) {
+                unrank_j(ra0, I - 1, n, binom, gv.M, Ap0);
 #pragma unroll
-            for (int t = 0; t < I - 1; ++t) Af[t] = Ap[t];
-            Af[I - 1] = Ap[I - 2];
-        } else {
-            Af[0] = 0;
+                for (int q = 0; q < I - 1; ++q) Af[q] = Ap0[q];
+                Af[I - 1] = Ap0[I - 2];
+            } else {
+                Af[0] = 0;
+            }
+            base_tr = rank_j(Af, I, n, binom, gv.M);
         }
-        out_row = static_cast<uint64_t>(rb) * st.cnt_a + rank_j(Af, I, n, binom, gv.M);
-    }
+    };
+    // the next tile of this workgroup's sequence; false at the end
+    auto advance = [&]() -> bool {
+        if (tix + 1 < ch_end) {
+            ++tix;
+            if (++rb == cnt_b) {  // next A'-range, first B
+                rb = 0;
+                ra0 += r;
+                if constexpr (J > 0) {
+#pragma unroll
+                    for (int q = 0; q < J; ++q) B[q] = 0;
+                    terms();
+                }
+                if (threadIdx.x == 0) {
+                    if constexpr (I > 1) {
+#pragma unroll
+                        for (int q = 0; q < I - 1; ++q) Ap0[q] = Ap[q];  // walked past the previous range
+                    }
+                    base_tr += last_total;
+                }
+            } else if constexpr (J > 0) {
+                ms_next<J>(B, n);
+                terms();
+            }
+            return true;
+        }
+        ch += gridDim.x;
+        if (ch >= nchunks) return false;
+        start_at(ch * st.chunk);
+        return true;
+    };
     // thread 0: the tile's rows t -> last element of A'_t and output offsets
     auto tile_table = [&](int b, uint32_t rc) {
+        if constexpr (I > 1) {
+#pragma unroll
+            for (int q = 0; q < I - 1; ++q) Ap[q] = Ap0[q];
+        }
         uint32_t pre = 0;
         for (uint32_t t = 0; t < rc; ++t) {
             const uint32_t lo = I > 1 ? Ap[I > 1 ? I - 2 : 0] : 0u;
@@ -704,13 +751,15 @@ __global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a,
             if constexpr (I > 1) ms_next<I - 1>(Ap, n);
         }
         t_pre[b][rc] = pre;
+        last_total = pre;
+        t_out[b] = static_cast<unsigned long long>(rb) * st.cnt_a + base_tr;
     };
     // the tile's input words (rows ra0.. of the n blocks) into registers
     uint64_t stage_w[kTileLoads];
     auto load_tile = [&](uint32_t rc) {
 #pragma unroll
         for (int m = 0; m < kTileLoads; ++m) {
-            const uint32_t i = threadIdx.x + m * kBlock;
+            const uint32_t i = threadIdx.x + m * kTileBlock;
             stage_w[m] = 0;
             if (i < words) {
                 const uint32_t v = i / rW, w = i - v * rW;
@@ -734,42 +783,24 @@ __global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a,
     auto store_tile = [&]() {
 #pragma unroll
         for (int m = 0; m < kTileLoads; ++m) {
-            const uint32_t i = threadIdx.x + m * kBlock;
+            const uint32_t i = threadIdx.x + m * kTileBlock;
             if (i < words) buf[i] = stage_w[m];
         }
     };
 
+    start_at(ch * st.chunk);
     uint32_t rc = min(r, cnt_in - ra0);
     if (threadIdx.x == 0) tile_table(0, rc);
     load_tile(rc);
     store_tile();
-    __shared__ unsigned long long t_out[2];
-    if (threadIdx.x == 0) t_out[0] = out_row;
-    for (uint32_t k = 0; k < count; ++k) {
+    for (uint32_t k = 0;; ++k) {
         const int cur = k & 1, nxt = cur ^ 1;
         __syncthreads();  // tile k is in buf
         const uint32_t rc_cur = rc;
-        const bool more = k + 1 < count;
-        if (more) {  // advance to tile k + 1 and start its loads
-            ra0 += r;
-            if (ra0 >= cnt_in) {
-                ra0 = 0;
-                ++rb;
-                if constexpr (J > 0) {
-                    ms_next<J>(B, n);
-                    terms();
-                }
-                if constexpr (I > 1) {
-                    if (threadIdx.x == 0)
-#pragma unroll
-                        for (int t = 0; t < I - 1; ++t) Ap[t] = 0;
-                }
-            }
+        const bool more = advance();  // uniform
+        if (more) {  // start the next tile's loads
             rc = min(r, cnt_in - ra0);
-            if (threadIdx.x == 0) {
-                tile_table(nxt, rc);
-                t_out[nxt] = t_out[cur] + t_pre[cur][rc_cur];
-            }
+            if (threadIdx.x == 0) tile_table(nxt, rc);
             load_tile(rc);
         }
         const uint64_t* tb = buf;
@@ -797,10 +828,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_tile(GraphView gv, LevelArgs a,
             }
             store_row<W>(st.out + o * W, acc);
         }
-        if (more) {  // every thread is done with tile k: the next one replaces it
-            __syncthreads();
-            store_tile();
-        }
+        if (!more) break;
+        __syncthreads();  // every thread is done with tile k: the next one replaces it
+        store_tile();
     }
 }
 
@@ -1638,7 +1668,8 @@ struct cr_solver {
     uint64_t ms_count[CR_MAX_K + 1] = {};     // C(n+j-1, j): multisets of size j
     bool pull_tiles = true;                   // k_pull_tile (else k_sep_stage) runs the pull stages
     uint32_t tile_rows = 4;                   // k_pull_tile: rows of each input block per tile
-    uint32_t tile_blocks = 8192;              // k_pull_tile: workgroups (each walks a chunk of tiles)
+    uint32_t tile_blocks = 8192;              // k_pull_tile: workgroups
+    uint32_t tile_chunk = 16;                 // k_pull_tile: consecutive tiles a workgroup takes at a time
     uint32_t pull_rows = 0xffffffffu;         // frontier rows above which a level pulls
     uint32_t push_blocks = 2048;
     uint32_t row_blocks = 2048;
@@ -1880,7 +1911,7 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
         // the row kernel) where a large graph image leaves too little room
         auto tile_lds = [&](uint32_t r) { return ((s->gv.image_bytes + 15) & ~15u) + s->g.n * r * W * 8; };
         auto fits = [&](uint32_t r) {
-            return tile_lds(r) <= kLdsBytes && s->g.n * r * W <= static_cast<uint32_t>(kTileLoads * kBlock);
+            return tile_lds(r) <= kLdsBytes && s->g.n * r * W <= static_cast<uint32_t>(kTileLoads * kTileBlock);
         };
         while (st.r > 1 && !fits(st.r)) --st.r;
         const bool tiles = s->pull_tiles && fits(st.r);
@@ -1899,14 +1930,15 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
         if (tiles) {
             lds = tile_lds(st.r);
             const uint64_t ntiles = static_cast<uint64_t>(st.cnt_b) * ((st.cnt_a_in + st.r - 1) / st.r);
-            blocks = std::min<uint64_t>(ntiles, s->tile_blocks);
+            st.chunk = s->tile_chunk;
+            blocks = std::min<uint64_t>((ntiles + st.chunk - 1) / st.chunk, s->tile_blocks);
         } else {
             blocks = std::min<uint64_t>(((st.rows + 63) / 64 + kWavesPerBlock - 1) / kWavesPerBlock, 16384);
         }
         if (int rc = raise_lds_limit(fn, lds)) return rc;
         blocks = std::max<uint64_t>(kXcds, (blocks + kXcds - 1) / kXcds * kXcds);
         Timer* t = start_timer(s, "sep_stage");
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), lds, s->stream, s->gv, a, st);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(tiles ? kTileBlock : kBlock), lds, s->stream, s->gv, a, st);
         t->stop();
         return launch_pull_stage<K, W, I + 1>(s, a, st.out);
     }
@@ -3090,6 +3122,10 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
         case CR_TUNE_PULL_TILE_BLOCKS:
             if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "tile blocks %lld", static_cast<long long>(value));
             s->tile_blocks = static_cast<uint32_t>(value);
+            return 0;
+        case CR_TUNE_PULL_TILE_CHUNK:
+            if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "tile chunk %lld", static_cast<long long>(value));
+            s->tile_chunk = static_cast<uint32_t>(value);
             return 0;
         case CR_TUNE_PULL_TILE_ROWS:
             if (value < 1 || value > kTileMaxRows) return fail(CR_ERR_ARG, "tile rows %lld", static_cast<long long>(value));

@@ -234,13 +234,14 @@ enum cr_tuning {
                                     bitsets fit in 256 MiB, else 190; a negative value restores it */
     CR_TUNE_PULL_TILES = 7,      /* 1 (default): the pull's stages load their input rows through LDS
                                     tiles; 0: one row per lane with direct gathers */
-    CR_TUNE_PULL_TILE_ROWS = 8,  /* rows of each input block per LDS tile, 1..16 (default 2) */
+    CR_TUNE_PULL_TILE_ROWS = 8,  /* rows of each input block per LDS tile, 1..16 (default 4) */
     CR_TUNE_PULL_TILE_BLOCKS = 9, /* workgroups of a tiled pull stage (default 8192) */
     CR_TUNE_DEPTH_LOG = 10,      /* capacity of the depth log in units of the configuration count
                                     (default 40; 0: no log, every depth writes its bytes directly):
                                     each depth's new states are logged and written to the table in row
                                     chunks. Takes effect at the next cr_build_aux_graph */
-    CR_TUNE_DEPTH_LOG_MIN = 11   /* configurations below which no depth log is kept (default 2^24) */
+    CR_TUNE_DEPTH_LOG_MIN = 11,  /* configurations below which no depth log is kept (default 2^24) */
+    CR_TUNE_PULL_TILE_CHUNK = 12 /* consecutive tiles a pull workgroup takes at a time (default 16) */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:
