@@ -28,9 +28,10 @@ bit) for the finaliser.
 
 cpu_baseline: the reference solver (oracle/_ref/k_cops_visibility, built from
 the reference's own sources) on one host core, on a bounded sample of the
-workload -- the same graph with one cop fewer -- as GTEPS of that run.
-cpu_port: the multithreaded level-synchronous CPU restatement of the oracle
-on the full workload, on the host's cores.
+workload -- the same graph with fewer cops, the most that keep the sample to
+at most 50,000 configurations (k = 2 for n = 200, ~2 s) -- as GTEPS of that
+run. cpu_port: the multithreaded level-synchronous CPU restatement of the
+oracle on the full workload, on the host's cores (skipped above 2e9 states).
 """
 import argparse
 import json
@@ -487,8 +488,13 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        # bounded sample: the reference on the same graph with one cop fewer
+        # bounded sample: the reference on the same graph with fewer cops, the
+        # most that keep it to at most 50,000 configurations (~2 s on one core
+        # for n = 200)
+        from math import comb
         sk = max(1, k - 1)
+        while sk > 1 and comb(info.n + sk - 1, sk) > 50_000:
+            sk -= 1
         with cra.Solver(0) as t:
             t.load_graph_file(path, sk, 1)
             t.build_aux_graph()
@@ -497,7 +503,12 @@ def main():
             se = sum(t.retro_edges()[:2])
         out["cpu_baseline"] = cpu_reference_sample(stem, sk, se)
     if args.cpu_port < 0:
-        args.cpu_port = 0 if args.no_cpu_baseline else min(16, os.cpu_count() or 1)
+        # the port solves the full workload in host memory: up to 2e9 states
+        too_big = info.num_states > 2_000_000_000
+        args.cpu_port = 0 if args.no_cpu_baseline or too_big else min(16, os.cpu_count() or 1)
+        if too_big and not args.no_cpu_baseline:
+            out["cpu_port"] = {"skipped": f"{info.num_states} states: the full workload does not fit the "
+                                          "CPU port's host tables within minutes"}
     if args.cpu_port:
         port, states = cpu_port(stem, k, args.cpu_port)
         out["cpu_port"] = port
