@@ -127,6 +127,7 @@ struct LevelArgs {
     uint32_t* log_rows;        // depth log: configuration of each entry
     uint64_t* log_bits;        // depth log: W robber words of each entry
     int32_t n, W;
+    int32_t fuse;              // pulled levels: the last pull stage does k_cop_finalize's work
 };
 
 // Work items [0, n) in 8 contiguous segments, one per XCD (workgroups are
@@ -432,6 +433,7 @@ struct SepStage {
     int32_t stage;        // i, 1..K
     uint32_t r;           // k_pull_tile: rows of each input block per tile
     uint32_t chunk;       // k_pull_tile: tiles a workgroup takes at a time
+    int32_t fuse;         // stage K: do k_cop_finalize's work instead of writing nx0
 };
 
 // Loads / stores of one W-word row: 16-byte vector accesses where the row is
@@ -611,6 +613,51 @@ __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a,
             for (int q = 0; q < W; ++q) acc[q] &= ~w0[q] & full[q];
         }
         store_row<W>(st.out + o * W, acc);
+    }
+}
+
+// The row flags and group counts of 64 consecutive configurations starting
+// at `base` (one wave; bit i of `bits` = configuration base + i has new
+// states, `pop` = this lane's new states), added into zeroed words with at
+// most two atomics each: 64 rows touch at most two flag words and two groups
+// of kGroupWords * 64 rows.
+__device__ __forceinline__ void fuse_flags(const LevelArgs& a, uint64_t base, uint64_t bits, uint32_t pop) {
+    const int lane = threadIdx.x & 63;
+    constexpr uint32_t kGroupRows = kGroupWords * 64;
+    const uint64_t g0 = base / kGroupRows;
+    const uint32_t split = static_cast<uint32_t>(min<uint64_t>(64, (g0 + 1) * kGroupRows - base));  // lanes in g0
+    const bool first = static_cast<uint32_t>(lane) < split;
+    unsigned long long s0 = first ? pop : 0u, s1 = first ? 0u : pop;
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if (lane != 0 || bits == 0) return;
+    const uint64_t w = base / 64;
+    const uint32_t sh = static_cast<uint32_t>(base % 64);
+    atomicOr(reinterpret_cast<unsigned long long*>(a.c0.flags + w), static_cast<unsigned long long>(bits << sh));
+    if (sh && (bits >> (64 - sh)))
+        atomicOr(reinterpret_cast<unsigned long long*>(a.c0.flags + w + 1),
+                 static_cast<unsigned long long>(bits >> (64 - sh)));
+    const uint64_t lo = split >= 64 ? bits : (bits & ((1ull << split) - 1));
+    const uint32_t c0 = static_cast<uint32_t>(__popcll(lo)), c1 = static_cast<uint32_t>(__popcll(bits)) - c0;
+    if (c0) {
+        atomicAdd(a.c0.gcnt + g0, c0);
+        atomicAdd(a.c0.gstates + g0, s0);
+    }
+    if (c1) {
+        atomicAdd(a.c0.gcnt + g0 + 1, c1);
+        atomicAdd(a.c0.gstates + g0 + 1, s1);
+    }
+}
+
+// Zeroes the cop-turn flags and group counts before a fused last pull stage.
+__global__ __launch_bounds__(kBlock) void k_zero_groups(LevelArgs a) {
+    if (a.ctl->done || !a.ctl->pull) return;
+    const uint32_t ngroups = (a.nbatch + kGroupWords - 1) / kGroupWords;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.nbatch; i += gridDim.x * blockDim.x)
+        a.c0.flags[i] = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ngroups; i += gridDim.x * blockDim.x) {
+        a.c0.gcnt[i] = 0;
+        a.c0.gstates[i] = 0;
     }
 }
 
@@ -806,27 +853,49 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArg
         const uint64_t* tb = buf;
         const uint32_t total = t_pre[cur][rc_cur];
         const uint64_t o0 = t_out[cur];
-        for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) {
-            uint32_t t = 0;
-            while (t + 1 < rc_cur && t_pre[cur][t + 1] <= j) ++t;
-            const uint32_t c = t_lo[cur][t] + (j - t_pre[cur][t]);
+        for (uint32_t j0 = 0; j0 < total; j0 += blockDim.x) {  // every lane runs every round (ballots)
+            const uint32_t j = j0 + threadIdx.x;
+            const bool valid = j < total;
             uint64_t acc[W];
 #pragma unroll
             for (int q = 0; q < W; ++q) acc[q] = 0;
-            const int beg = out_off[c], end = out_off[c + 1];
-            for (int e = beg; e < end; ++e) {
-                const uint64_t* src = tb + out_nbr[e] * rW + t * W;
+            if (valid) {
+                uint32_t t = 0;
+                while (t + 1 < rc_cur && t_pre[cur][t + 1] <= j) ++t;
+                const uint32_t c = t_lo[cur][t] + (j - t_pre[cur][t]);
+                const int beg = out_off[c], end = out_off[c + 1];
+                for (int e = beg; e < end; ++e) {
+                    const uint64_t* src = tb + out_nbr[e] * rW + t * W;
 #pragma unroll
-                for (int q = 0; q < W; ++q) acc[q] |= src[q];
+                    for (int q = 0; q < W; ++q) acc[q] |= src[q];
+                }
             }
             const uint64_t o = o0 + j;
             if constexpr (J == 0) {  // candidates of configuration o: only states not won yet
                 uint64_t w0[W];
-                load_row<W>(a.won0 + o * W, w0);
+                uint32_t pop = 0;
+                if (valid) {
+                    load_row<W>(a.won0 + o * W, w0);
 #pragma unroll
-                for (int q = 0; q < W; ++q) acc[q] &= ~w0[q] & full[q];
+                    for (int q = 0; q < W; ++q) {
+                        acc[q] &= ~w0[q] & full[q];
+                        pop += __popcll(acc[q]);
+                    }
+                }
+                if (st.fuse) {
+                    // k_cop_finalize's work for a pulled level: only rows with
+                    // new states are written (nx0 is zero elsewhere), and the
+                    // row flags and group counts (zeroed before the stage) take
+                    // this wave's rows. won0 and fr0 are updated after the
+                    // level's robber step, by k_log_lists.
+                    if (pop) store_row<W>(st.out + o * W, acc);
+                    fuse_flags(a, o0 + j0 + (threadIdx.x & ~63u), __ballot(pop != 0), pop);
+                } else if (valid) {
+                    store_row<W>(st.out + o * W, acc);
+                }
+            } else if (valid) {
+                store_row<W>(st.out + o * W, acc);
             }
-            store_row<W>(st.out + o * W, acc);
         }
         if (!more) break;
         __syncthreads();  // every thread is done with tile k: the next one replaces it
@@ -877,11 +946,18 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
             const uint32_t row = live ? list[i] : 0u;
             const uint64_t o = static_cast<uint64_t>(row) * W;
             uint64_t w0[W], w1[W], f0[W], nw[W], cand[W];
+            if (live) {  // 16-byte row loads (rows are 16-byte aligned for even W)
+                load_row<W>(a.won0 + o, w0);
+                load_row<W>(a.won1 + o, w1);
+                load_row<W>(a.fr0 + o, f0);
+            }
 #pragma unroll
             for (int q = 0; q < W; ++q) {
-                w0[q] = live ? a.won0[o + q] : 0ull;
-                w1[q] = live ? a.won1[o + q] : ~0ull;
-                f0[q] = live ? a.fr0[o + q] : 0ull;
+                if (!live) {
+                    w0[q] = 0;
+                    w1[q] = ~0ull;
+                    f0[q] = 0;
+                }
                 nw[q] = 0;
                 cand[q] = 0;
             }
@@ -915,10 +991,13 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
             for (int q = 0; q < W; ++q) pop += __popcll(nw[q]);
             const bool append = pop != 0;
             if (append) {
+                uint64_t u[W];
+#pragma unroll
+                for (int q = 0; q < W; ++q) u[q] = w1[q] | nw[q];
+                store_row<W>(a.won1 + o, u);
+                store_row<W>(a.fr1 + o, nw);
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
-                    a.won1[o + q] = w1[q] | nw[q];
-                    a.fr1[o + q] = nw[q];
                     if (a.log_rows) continue;  // the depth bytes go through the log (k_log_lists)
                     uint64_t bits = nw[q];
                     while (bits) {
@@ -948,7 +1027,7 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs a) {
     const int lane = threadIdx.x & 63;
-    if (a.ctl->done) return;
+    if (a.ctl->done || (a.fuse && a.ctl->pull)) return;  // fused into the last pull stage
     const uint8_t nb = won_byte(a.ctl->level + 1);
     const uint32_t ngroups = (a.nbatch + kGroupWords - 1) / kGroupWords;
     const uint32_t wid = wave_id();
@@ -1183,7 +1262,9 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
 // The depth bytes of the states listed by k_write_lists (not in init mode):
 // one thread per list entry takes the row's new robber bits (fr1 for the
 // robber turn, fr0 for the cop turn) into the depth log, or, when the log has
-// no room for this depth, writes the bytes into the table directly.
+// no room for this depth, writes the bytes into the table directly. After a
+// pulled level with a fused last stage the cop-turn bits are still in nx0:
+// they are settled into won0 and fr0 here.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
     const Ctl* ctl = a.ctl;
@@ -1191,6 +1272,7 @@ __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
     const uint32_t depth = ctl->level, nxt = depth & 1;
     const uint32_t n0 = ctl->cnt0[nxt], n1 = ctl->cnt1[nxt];
     const bool logging = ctl->log_now != 0;
+    const bool fused = a.fuse && ctl->dir[depth - 1] != 0;  // the level that listed these states pulled
     const uint8_t nb = won_byte(depth);
     const unsigned long long off0 = ctl->log_off[depth][0], off1 = ctl->log_off[depth][1];
     const uint64_t total = static_cast<uint64_t>(n0) + n1;
@@ -1199,9 +1281,25 @@ __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
         const bool robber = i >= n0;
         const uint32_t k = static_cast<uint32_t>(robber ? i - n0 : i);
         const uint32_t row = (robber ? a.list1[nxt] : a.list0[nxt])[k];
-        const uint64_t* src = (robber ? a.fr1 : a.fr0) + static_cast<uint64_t>(row) * W;
+        const uint64_t o = static_cast<uint64_t>(row) * W;
         uint64_t w[W];
-        load_row<W>(src, w);
+        if (!robber && fused) {
+            // a pulled level whose last stage left the new cop-turn states in
+            // nx0: settle them now that the level's robber step has run
+            uint64_t w0[W], z[W];
+            load_row<W>(a.nx0 + o, w);
+            load_row<W>(a.won0 + o, w0);
+#pragma unroll
+            for (int q = 0; q < W; ++q) {
+                w0[q] |= w[q];
+                z[q] = 0;
+            }
+            store_row<W>(a.won0 + o, w0);
+            store_row<W>(a.fr0 + o, w);
+            store_row<W>(a.nx0 + o, z);
+        } else {
+            load_row<W>((robber ? a.fr1 : a.fr0) + o, w);
+        }
         if (logging) {
             const unsigned long long e = (robber ? off1 : off0) + k;
             a.log_rows[e] = row;
@@ -1670,6 +1768,8 @@ struct cr_solver {
     uint32_t tile_rows = 4;                   // k_pull_tile: rows of each input block per tile
     uint32_t tile_blocks = 8192;              // k_pull_tile: workgroups
     uint32_t tile_chunk = 16;                 // k_pull_tile: consecutive tiles a workgroup takes at a time
+    bool fuse_finalize = true;                // tuning: with the depth log, pulled levels finalize in the last stage
+    bool fuse_ok = false;                     // the last pull stage runs as LDS tiles (image and one row per vertex fit)
     uint32_t pull_rows = 0xffffffffu;         // frontier rows above which a level pulls
     uint32_t push_blocks = 2048;
     uint32_t row_blocks = 2048;
@@ -1915,6 +2015,10 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
         };
         while (st.r > 1 && !fits(st.r)) --st.r;
         const bool tiles = s->pull_tiles && fits(st.r);
+        st.fuse = I == K && tiles && a.fuse;
+        if (I == K && a.fuse && !st.fuse) return fail(CR_ERR_DEVICE, "fused finalize without the tiled last stage");
+        if (st.fuse)
+            hipLaunchKernelGGL(k_zero_groups, dim3(1024), dim3(kBlock), 0, s->stream, a);
         auto kern = tiles ? k_pull_tile<I, K - I, W> : k_sep_stage<I, K - I, W>;
         const void* fn = reinterpret_cast<const void*>(kern);
         st.stage = I;
@@ -2017,6 +2121,7 @@ LevelArgs level_args(cr_solver* s) {
     a.log_bits = s->d_log_bits;
     a.n = s->g.n;
     a.W = s->W;
+    a.fuse = s->d_log_rows != nullptr && s->pull_tiles && s->fuse_finalize && s->fuse_ok;
     for (int b = 0; b < 2; ++b) {
         a.list0[b] = s->d_list[0][b];
         a.list1[b] = s->d_list[1][b];
@@ -2194,6 +2299,8 @@ int alloc_log(cr_solver* s) {
     if (!s->h_lists)
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_lists), kMaxLogLists * sizeof(LogList), hipHostMallocDefault));
     s->log_cap = cap;
+    s->fuse_ok = ((s->gv.image_bytes + 15) & ~15u) + static_cast<uint64_t>(s->g.n) * s->W * 8 <= kLdsBytes &&
+                 static_cast<uint32_t>(s->g.n * s->W) <= static_cast<uint32_t>(kTileLoads * kTileBlock);
     return 0;
 }
 
@@ -3123,6 +3230,7 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "tile blocks %lld", static_cast<long long>(value));
             s->tile_blocks = static_cast<uint32_t>(value);
             return 0;
+        case CR_TUNE_FUSE_FINALIZE: s->fuse_finalize = value != 0; return 0;
         case CR_TUNE_PULL_TILE_CHUNK:
             if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "tile chunk %lld", static_cast<long long>(value));
             s->tile_chunk = static_cast<uint32_t>(value);
