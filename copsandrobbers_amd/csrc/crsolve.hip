@@ -434,6 +434,7 @@ struct SepStage {
     uint32_t r;           // k_pull_tile: rows of each input block per tile
     uint32_t chunk;       // k_pull_tile: tiles a workgroup takes at a time
     int32_t fuse;         // stage K: do k_cop_finalize's work instead of writing nx0
+    int32_t clear;        // stage 2: clear the frontier rows of fr1 (k_clear_frontier's work)
 };
 
 // Loads / stores of one W-word row: 16-byte vector accesses where the row is
@@ -712,6 +713,14 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArg
     // repeats find it in the caches.
     const uint64_t nchunks = (ntiles + st.chunk - 1) / st.chunk;
     uint64_t ch = blockIdx.x;
+    if (st.clear) {  // stage 2: stage 1 has read the frontier; clear its rows for the robber step
+        const uint32_t cur = a.ctl->level & 1;
+        const uint64_t cw = static_cast<uint64_t>(a.ctl->cnt1[cur]) * W;
+        const uint32_t* list = a.list1[cur];
+        for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < cw;
+             i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+            a.fr1[static_cast<uint64_t>(list[i / W]) * W + i % W] = 0;
+    }
     if (ch >= nchunks) return;  // uniform per workgroup
 
     // walk state (uniform): the tile, its B with rank terms and first A' row;
@@ -1770,6 +1779,7 @@ struct cr_solver {
     uint32_t tile_chunk = 16;                 // k_pull_tile: consecutive tiles a workgroup takes at a time
     bool fuse_finalize = true;                // tuning: with the depth log, pulled levels finalize in the last stage
     bool fuse_ok = false;                     // the last pull stage runs as LDS tiles (image and one row per vertex fit)
+    bool stage2_cleared = false;              // the pull being enqueued clears the frontier in stage 2
     uint32_t pull_rows = 0xffffffffu;         // frontier rows above which a level pulls
     uint32_t push_blocks = 2048;
     uint32_t row_blocks = 2048;
@@ -2016,6 +2026,8 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
         while (st.r > 1 && !fits(st.r)) --st.r;
         const bool tiles = s->pull_tiles && fits(st.r);
         st.fuse = I == K && tiles && a.fuse;
+        st.clear = I == 2 && tiles;
+        if (st.clear) s->stage2_cleared = true;
         if (I == K && a.fuse && !st.fuse) return fail(CR_ERR_DEVICE, "fused finalize without the tiled last stage");
         if (st.fuse)
             hipLaunchKernelGGL(k_zero_groups, dim3(1024), dim3(kBlock), 0, s->stream, a);
@@ -2050,7 +2062,9 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
 
 template <int K, int W>
 int launch_pull(cr_solver* s, LevelArgs a) {
+    s->stage2_cleared = false;
     if (int rc = launch_pull_stage<K, W, 1>(s, a, s->d_fr1)) return rc;
+    if (s->stage2_cleared) return 0;  // the tiled stage 2 cleared the frontier
     Timer* t = start_timer(s, "clear_frontier");
     hipLaunchKernelGGL(k_clear_frontier, dim3(1024), dim3(kBlock), 0, s->stream, a, s->W);
     t->stop();
