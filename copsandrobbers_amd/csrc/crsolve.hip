@@ -912,6 +912,134 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArg
     }
 }
 
+// Stage 1 of the pull (A' empty, J = K - 1 >= 1) by tiles of r consecutive
+// B: H_1[B][c] = OR_{v in N(c)} fr1[B + v]. A tile loads the rows B_t + v
+// (t < r, every v) into LDS as tile[v][t]. Consecutive B differ only in
+// their last element while it does not wrap, so for every v <= max B the
+// rows B_t + v of consecutive t are adjacent: the tile's reads come in runs
+// of r rows instead of single rows. Its outputs, rows rank(B_t) * n + c, are
+// r * n consecutive rows of H_1. Thread x always loads word x mod W of B_t,
+// t = (x mod rW) / W, for rows v = x / rW + m * (blockDim / rW), so each
+// thread keeps one B_t and its rank terms. Same chunked walk and register
+// staging as k_pull_tile.
+template <int J, int W>
+__global__ __launch_bounds__(kTileBlock) void k_pull_tile1(GraphView gv, LevelArgs a, SepStage st) {
+    static_assert(J > 0, "stage 1 of a single-cop pull is its last stage");
+    extern __shared__ __align__(16) uint8_t lds[];
+    if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
+    load_image(lds, gv.image, gv.image_bytes);
+    const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
+    const uint16_t* out_off = reinterpret_cast<const uint16_t*>(lds + gv.off_out_off);
+    const uint8_t* out_nbr = lds + gv.off_out_nbr;
+    const uint32_t n = gv.n;
+    const uint32_t Mp = gv.M + 1;
+    const uint32_t r = st.r, rW = r * W;
+    const uint32_t per = kTileBlock / rW;  // input rows v that one pass of the workgroup loads
+    const bool loader = threadIdx.x < per * rW;
+    const uint32_t tq = threadIdx.x % rW, my_t = tq / W, my_w = tq % W, v0 = threadIdx.x / rW;
+    uint64_t* buf = reinterpret_cast<uint64_t*>(lds + ((gv.image_bytes + 15) & ~15u));
+    const uint32_t cnt_b = st.cnt_b;
+    const uint64_t ntiles = (cnt_b + r - 1) / r;
+    const uint64_t nchunks = (ntiles + st.chunk - 1) / st.chunk;
+    uint64_t ch = blockIdx.x;
+    if (ch >= nchunks) return;  // uniform per workgroup
+
+    constexpr int J1 = J + 1;
+    const uint32_t M1 = n + J;  // M of (J+1)-multisets
+    const uint32_t rank0 = binom[J1 * Mp + M1] - 1;
+    uint64_t tix = 0, ch_end = 0;
+    uint32_t rb0 = 0;              // rank of the tile's first B
+    uint32_t B[J];                 // the tile's first B
+    uint32_t Bt[J], U[J], S[J];    // this thread's B_t and its rank terms
+    bool mine = false;             // B_t exists
+    auto terms = [&]() {
+#pragma unroll
+        for (int q = 0; q < J; ++q) Bt[q] = B[q];
+        mine = rb0 + my_t < cnt_b;
+        if (mine)
+            for (uint32_t t = 0; t < my_t; ++t) ms_next<J>(Bt, n);
+#pragma unroll
+        for (int t = 0; t < J; ++t) {
+            U[t] = binom[(J1 - t) * Mp + (M1 - 1 - Bt[t] - t)];
+            S[t] = binom[(J - t) * Mp + (M1 - 2 - Bt[t] - t)];
+        }
+    };
+    auto start_at = [&](uint64_t t) {  // walk state from scratch at tile t
+        tix = t;
+        ch_end = min(ntiles, (t / st.chunk + 1) * st.chunk);
+        rb0 = static_cast<uint32_t>(t * r);
+        unrank_j(rb0, J, n, binom, gv.M, B);
+        terms();
+    };
+    auto advance = [&]() -> bool {  // the next tile of this workgroup's sequence; false at the end
+        if (tix + 1 < ch_end) {
+            ++tix;
+            for (uint32_t t = 0; t < r; ++t) ms_next<J>(B, n);
+            rb0 += r;
+            terms();
+            return true;
+        }
+        ch += gridDim.x;
+        if (ch >= nchunks) return false;
+        start_at(ch * st.chunk);
+        return true;
+    };
+    uint64_t stage_w[kTileLoads];
+    auto load_tile = [&]() {  // rows B_t + v of this thread's B_t, word my_w
+#pragma unroll
+        for (int m = 0; m < kTileLoads; ++m) {
+            const uint32_t v = v0 + m * per;
+            stage_w[m] = 0;
+            if (loader && mine && v < n) {
+                uint32_t sum = 0, q = 0;
+#pragma unroll
+                for (int t = 0; t < J; ++t) {
+                    const bool lt = Bt[t] < v;
+                    sum += lt ? U[t] : S[t];
+                    q += lt ? 1u : 0u;
+                }
+                const uint32_t idx = rank0 - sum - binom[(J1 - q) * Mp + (M1 - 1 - v - q)];
+                stage_w[m] = st.in[static_cast<uint64_t>(idx) * W + my_w];
+            }
+        }
+    };
+    auto store_tile = [&]() {
+#pragma unroll
+        for (int m = 0; m < kTileLoads; ++m) {
+            const uint32_t v = v0 + m * per;
+            if (loader && v < n) buf[v * rW + tq] = stage_w[m];
+        }
+    };
+
+    start_at(ch * st.chunk);
+    load_tile();
+    store_tile();
+    for (;;) {
+        __syncthreads();  // the current tile is in buf
+        const uint32_t rc = min(r, cnt_b - rb0);
+        const uint64_t o0 = static_cast<uint64_t>(rb0) * n;
+        const bool more = advance();  // uniform
+        if (more) load_tile();        // the next tile's loads stay in flight during the compute
+        const uint32_t total = rc * n;
+        for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) {
+            const uint32_t t = j / n, c = j - t * n;
+            uint64_t acc[W];
+#pragma unroll
+            for (int q = 0; q < W; ++q) acc[q] = 0;
+            const int beg = out_off[c], end = out_off[c + 1];
+            for (int e = beg; e < end; ++e) {
+                const uint64_t* src = buf + out_nbr[e] * rW + t * W;
+#pragma unroll
+                for (int q = 0; q < W; ++q) acc[q] |= src[q];
+            }
+            store_row<W>(st.out + (o0 + j) * W, acc);
+        }
+        if (!more) break;
+        __syncthreads();  // every thread is done with the current tile: the next one replaces it
+        store_tile();
+    }
+}
+
 // After a pull, the frontier rows still hold their depth-d robber bits;
 // clear them before the robber step writes depth d+1 into its rows.
 __global__ __launch_bounds__(kBlock) void k_clear_frontier(LevelArgs a, int32_t W) {
@@ -1777,6 +1905,7 @@ struct cr_solver {
     uint32_t tile_rows = 4;                   // k_pull_tile: rows of each input block per tile
     uint32_t tile_blocks = 8192;              // k_pull_tile: workgroups
     uint32_t tile_chunk = 16;                 // k_pull_tile: consecutive tiles a workgroup takes at a time
+    bool tile1 = true;                        // k_pull_tile1 runs stage 1 of a tiled multi-cop pull
     bool fuse_finalize = true;                // tuning: with the depth log, pulled levels finalize in the last stage
     bool fuse_ok = false;                     // the last pull stage runs as LDS tiles (image and one row per vertex fit)
     bool stage2_cleared = false;              // the pull being enqueued clears the frontier in stage 2
@@ -2007,6 +2136,15 @@ struct LaunchPush {
     }
 };
 
+// k_pull_tile1 exists for J >= 1; the J = 0 branch is never launched.
+template <int J, int W>
+auto pull_tile1_kernel() {
+    if constexpr (J > 0)
+        return k_pull_tile1<J, W>;
+    else
+        return k_pull_tile<1, 0, W>;
+}
+
 // Stages I..K of one pull; stage I reads `in` (H_{I-1}) and writes H_I.
 // The grid is a multiple of the XCD count (k_sep_stage splits its windows
 // per XCD).
@@ -2016,12 +2154,17 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
         return 0;
     } else {
         SepStage st{};
-        st.r = I == 1 ? 1u : s->tile_rows;
-        // LDS of a tile: image, base[n] and n * r rows; fewer rows per tile (or
-        // the row kernel) where a large graph image leaves too little room
+        // k_pull_tile1 takes stage 1 of a multi-cop pull (r consecutive B per tile)
+        const bool tile1 = I == 1 && K > 1 && s->tile1;
+        st.r = I == 1 && !tile1 ? 1u : s->tile_rows;
+        // LDS of a tile: image and n * r rows; fewer rows per tile (or the row
+        // kernel) where a large graph image leaves too little room
         auto tile_lds = [&](uint32_t r) { return ((s->gv.image_bytes + 15) & ~15u) + s->g.n * r * W * 8; };
         auto fits = [&](uint32_t r) {
-            return tile_lds(r) <= kLdsBytes && s->g.n * r * W <= static_cast<uint32_t>(kTileLoads * kTileBlock);
+            if (tile_lds(r) > kLdsBytes || r * W > static_cast<uint32_t>(kTileBlock)) return false;
+            if (tile1)  // each thread loads kTileLoads rows v of one B_t
+                return s->g.n <= static_cast<uint32_t>(kTileLoads) * (kTileBlock / (r * W));
+            return s->g.n * r * W <= static_cast<uint32_t>(kTileLoads * kTileBlock);
         };
         while (st.r > 1 && !fits(st.r)) --st.r;
         const bool tiles = s->pull_tiles && fits(st.r);
@@ -2031,7 +2174,7 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
         if (I == K && a.fuse && !st.fuse) return fail(CR_ERR_DEVICE, "fused finalize without the tiled last stage");
         if (st.fuse)
             hipLaunchKernelGGL(k_zero_groups, dim3(1024), dim3(kBlock), 0, s->stream, a);
-        auto kern = tiles ? k_pull_tile<I, K - I, W> : k_sep_stage<I, K - I, W>;
+        auto kern = !tiles ? k_sep_stage<I, K - I, W> : tile1 ? pull_tile1_kernel<K - I, W>() : k_pull_tile<I, K - I, W>;
         const void* fn = reinterpret_cast<const void*>(kern);
         st.stage = I;
         st.in = in;
@@ -2045,7 +2188,8 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
         uint64_t blocks = 0;
         if (tiles) {
             lds = tile_lds(st.r);
-            const uint64_t ntiles = static_cast<uint64_t>(st.cnt_b) * ((st.cnt_a_in + st.r - 1) / st.r);
+            const uint64_t ntiles = tile1 ? (st.cnt_b + st.r - 1) / st.r
+                                          : static_cast<uint64_t>(st.cnt_b) * ((st.cnt_a_in + st.r - 1) / st.r);
             st.chunk = s->tile_chunk;
             blocks = std::min<uint64_t>((ntiles + st.chunk - 1) / st.chunk, s->tile_blocks);
         } else {
@@ -3245,6 +3389,7 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             s->tile_blocks = static_cast<uint32_t>(value);
             return 0;
         case CR_TUNE_FUSE_FINALIZE: s->fuse_finalize = value != 0; return 0;
+        case CR_TUNE_PULL_TILE1: s->tile1 = value != 0; return 0;
         case CR_TUNE_PULL_TILE_CHUNK:
             if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "tile chunk %lld", static_cast<long long>(value));
             s->tile_chunk = static_cast<uint32_t>(value);

@@ -242,8 +242,10 @@ enum cr_tuning {
                                     chunks. Takes effect at the next cr_build_aux_graph */
     CR_TUNE_DEPTH_LOG_MIN = 11,  /* configurations below which no depth log is kept (default 2^24) */
     CR_TUNE_PULL_TILE_CHUNK = 12, /* consecutive tiles a pull workgroup takes at a time (default 16) */
-    CR_TUNE_FUSE_FINALIZE = 13   /* 1 (default): where the depth log is kept, a pulled level's last
+    CR_TUNE_FUSE_FINALIZE = 13,  /* 1 (default): where the depth log is kept, a pulled level's last
                                     stage settles the new cop-turn states itself (no finalize pass) */
+    CR_TUNE_PULL_TILE1 = 14      /* 1 (default): a tiled pull's first stage takes CR_TUNE_PULL_TILE_ROWS
+                                    consecutive cop multisets per tile (else one per tile) */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:
