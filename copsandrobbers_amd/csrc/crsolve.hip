@@ -1906,6 +1906,7 @@ struct cr_solver {
     uint32_t tile_blocks = 8192;              // k_pull_tile: workgroups
     uint32_t tile_chunk = 16;                 // k_pull_tile: consecutive tiles a workgroup takes at a time
     bool tile1 = true;                        // k_pull_tile1 runs stage 1 of a tiled multi-cop pull
+    uint32_t pull_gate = 16;                  // a chunk may pull if its first frontier * pull_gate > pull_rows
     bool fuse_finalize = true;                // tuning: with the depth log, pulled levels finalize in the last stage
     bool fuse_ok = false;                     // the last pull stage runs as LDS tiles (image and one row per vertex fit)
     bool stage2_cleared = false;              // the pull being enqueued clears the frontier in stage 2
@@ -2841,20 +2842,33 @@ int do_loop(cr_solver* s) {
     while (!done) {
         std::vector<std::unique_ptr<Timer>> timers;
         s->timers = &timers;
+        // A chunk of levels that starts with a robber frontier far below the
+        // pull threshold pushes every level and enqueues no pull launches
+        // (on push levels they only return, at several microseconds each):
+        // its k_advance sees an unreachable threshold. Any level already set
+        // to pull keeps its pull.
+        LevelArgs la = a;
+        bool pulls = s->d_sep[0] != nullptr;
+        if (pulls && s->pull_gate > 0) {
+            const Ctl* h = s->h_ctl;
+            const uint64_t frontier = h->cnt1[h->level & 1];
+            pulls = h->pull != 0 || frontier * static_cast<uint64_t>(s->pull_gate) > s->pull_rows;
+        }
+        if (!pulls) la.pull_rows = 0xffffffffu;
         for (int l = 0; l < s->levels_per_sync; ++l) {
             Timer* t = start_timer(s, "cop_push");
-            if ((rc = dispatch_k<LaunchPush>(s->k, s, a)) != 0) return rc;
+            if ((rc = dispatch_k<LaunchPush>(s->k, s, la)) != 0) return rc;
             t->stop();
-            if (s->d_sep[0] && (rc = dispatch_k<LaunchPull>(s->k, s, a)) != 0) return rc;
+            if (pulls && (rc = dispatch_k<LaunchPull>(s->k, s, la)) != 0) return rc;
             t = start_timer(s, "robber_step");
-            if ((rc = dispatch_w<LaunchRobber>(s->W, s, a)) != 0) return rc;
+            if ((rc = dispatch_w<LaunchRobber>(s->W, s, la)) != 0) return rc;
             t->stop();
             t = start_timer(s, "cop_finalize");
-            if ((rc = dispatch_w<LaunchFinalize>(s->W, s, a)) != 0) return rc;
+            if ((rc = dispatch_w<LaunchFinalize>(s->W, s, la)) != 0) return rc;
             t->stop();
-            hipLaunchKernelGGL(k_advance, dim3(1), dim3(kScanThreads), 0, s->stream, a);
-            hipLaunchKernelGGL(k_write_lists, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, a, 0);
-            if (s->log_cap && (rc = dispatch_w<LaunchLogLists>(s->W, s, a)) != 0) return rc;
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(kScanThreads), 0, s->stream, la);
+            hipLaunchKernelGGL(k_write_lists, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, la, 0);
+            if (s->log_cap && (rc = dispatch_w<LaunchLogLists>(s->W, s, la)) != 0) return rc;
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl, offsetof(Ctl, lvl), hipMemcpyDeviceToHost, s->stream));
@@ -3390,6 +3404,10 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             return 0;
         case CR_TUNE_FUSE_FINALIZE: s->fuse_finalize = value != 0; return 0;
         case CR_TUNE_PULL_TILE1: s->tile1 = value != 0; return 0;
+        case CR_TUNE_PULL_GATE:
+            if (value < 0 || value > (1 << 20)) return fail(CR_ERR_ARG, "pull gate %lld", static_cast<long long>(value));
+            s->pull_gate = static_cast<uint32_t>(value);
+            return 0;
         case CR_TUNE_PULL_TILE_CHUNK:
             if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "tile chunk %lld", static_cast<long long>(value));
             s->tile_chunk = static_cast<uint32_t>(value);

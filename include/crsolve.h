@@ -244,8 +244,11 @@ enum cr_tuning {
     CR_TUNE_PULL_TILE_CHUNK = 12, /* consecutive tiles a pull workgroup takes at a time (default 16) */
     CR_TUNE_FUSE_FINALIZE = 13,  /* 1 (default): where the depth log is kept, a pulled level's last
                                     stage settles the new cop-turn states itself (no finalize pass) */
-    CR_TUNE_PULL_TILE1 = 14      /* 1 (default): a tiled pull's first stage takes CR_TUNE_PULL_TILE_ROWS
+    CR_TUNE_PULL_TILE1 = 14,     /* 1 (default): a tiled pull's first stage takes CR_TUNE_PULL_TILE_ROWS
                                     consecutive cop multisets per tile (else one per tile) */
+    CR_TUNE_PULL_GATE = 15       /* levels enqueued between host synchronisations may pull only if the
+                                    first one's robber frontier is above 1/gate of the pull threshold
+                                    (else they push and launch no pull kernels); 0 = always (default 16) */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:
