@@ -1177,22 +1177,24 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
             const uint64_t o = static_cast<uint64_t>(row) * W;
             uint64_t x[W];
             bool any = false;
+            if (row < gv.C) {
+                load_row<W>(a.nx0 + o, x);
 #pragma unroll
-            for (int q = 0; q < W; ++q) {
-                x[q] = row < gv.C ? a.nx0[o + q] : 0ull;
-                any |= x[q] != 0;
+                for (int q = 0; q < W; ++q) any |= x[q] != 0;
             }
             uint32_t pop = 0;
             if (any) {  // this lane's row has candidates: settle them
-                uint64_t nw[W];
+                uint64_t nw[W], wv[W], z[W];
+                load_row<W>(a.won0 + o, wv);
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
-                    const uint64_t wv = a.won0[o + q];
-                    nw[q] = x[q] & ~wv;
+                    nw[q] = x[q] & ~wv[q];
                     pop += __popcll(nw[q]);
-                    if (nw[q]) a.won0[o + q] = wv | nw[q];
-                    a.nx0[o + q] = 0;
+                    wv[q] |= nw[q];
+                    z[q] = 0;
                 }
+                if (pop) store_row<W>(a.won0 + o, wv);
+                store_row<W>(a.nx0 + o, z);
                 if (pop) {  // a listed row: its new states for the robber step and k_log_lists
 #pragma unroll
                     for (int q = 0; q < W; ++q) {
