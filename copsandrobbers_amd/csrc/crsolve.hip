@@ -1229,54 +1229,88 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
 // shuffles and one pass over the wave totals.
 constexpr int kScanPer = 8;
 
-__device__ void scan_groups(uint32_t* gcnt, const unsigned long long* gstates, uint32_t ng,
-                            uint32_t& total, unsigned long long& states) {
+// L lists of group counts are scanned in the same passes, so their loads are
+// in flight together (k_advance scans the cop-turn and robber-turn lists).
+template <int L>
+__device__ void scan_groups_n(uint32_t* const* gcnt, const unsigned long long* const* gstates, const uint32_t* ng,
+                              uint32_t* total, unsigned long long* states) {
     constexpr int kWaves = kScanThreads / 64;
-    __shared__ uint32_t wsum[kWaves];
-    __shared__ unsigned long long wst[kWaves];
+    __shared__ uint32_t wsum[L][kWaves];
+    __shared__ unsigned long long wst[L][kWaves];
     const uint32_t t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    uint32_t run = 0;
-    unsigned long long st = 0;
-    for (uint32_t base = 0; base < ng; base += kScanThreads * kScanPer) {
+    uint32_t run[L];
+    unsigned long long st[L];
+    uint32_t ngmax = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        run[l] = 0;
+        st[l] = 0;
+        ngmax = max(ngmax, ng[l]);
+    }
+    for (uint32_t base = 0; base < ngmax; base += kScanThreads * kScanPer) {
         const uint32_t i0 = base + t * kScanPer;
-        uint32_t v[kScanPer];
-        uint32_t sum = 0;
+        uint32_t v[L][kScanPer], sum[L], incl[L];
 #pragma unroll
-        for (int e = 0; e < kScanPer; ++e) {
-            v[e] = i0 + e < ng ? gcnt[i0 + e] : 0u;
-            st += i0 + e < ng ? gstates[i0 + e] : 0ull;
-            sum += v[e];
+        for (int l = 0; l < L; ++l) {
+            sum[l] = 0;
+#pragma unroll
+            for (int e = 0; e < kScanPer; ++e) {
+                const bool in = i0 + e < ng[l];
+                v[l][e] = in ? gcnt[l][i0 + e] : 0u;
+                st[l] += in ? gstates[l][i0 + e] : 0ull;
+                sum[l] += v[l][e];
+            }
+            incl[l] = sum[l];
         }
-        uint32_t incl = sum;
         for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t u = __shfl_up(incl, off);
-            if (lane >= static_cast<uint32_t>(off)) incl += u;
-        }
-        if (lane == 63) wsum[wid] = incl;
-        __syncthreads();
-        uint32_t pre = 0, tile = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            if (static_cast<uint32_t>(w) < wid) pre += wsum[w];
-            tile += wsum[w];
-        }
-        uint32_t r = run + pre + incl - sum;
 #pragma unroll
-        for (int e = 0; e < kScanPer; ++e) {
-            if (i0 + e < ng) gcnt[i0 + e] = r;
-            r += v[e];
+            for (int l = 0; l < L; ++l) {
+                const uint32_t u = __shfl_up(incl[l], off);
+                if (lane >= static_cast<uint32_t>(off)) incl[l] += u;
+            }
         }
-        run += tile;
+        if (lane == 63) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) wsum[l][wid] = incl[l];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            uint32_t pre = 0, tile = 0;
+            for (int w = 0; w < kWaves; ++w) {
+                if (static_cast<uint32_t>(w) < wid) pre += wsum[l][w];
+                tile += wsum[l][w];
+            }
+            uint32_t r = run[l] + pre + incl[l] - sum[l];
+#pragma unroll
+            for (int e = 0; e < kScanPer; ++e) {
+                if (i0 + e < ng[l]) gcnt[l][i0 + e] = r;
+                r += v[l][e];
+            }
+            run[l] += tile;
+        }
         __syncthreads();  // wsum is reused by the next tile
     }
-    st = wave_sum(st);
-    if (lane == 0) wst[wid] = st;
-    __syncthreads();
-    if (t == 0) {
-        total = run;
-        states = 0;
-        for (int w = 0; w < kWaves; ++w) states += wst[w];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        st[l] = wave_sum(st[l]);
+        if (lane == 0) wst[l][wid] = st[l];
     }
     __syncthreads();
+    if (t == 0) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            total[l] = run[l];
+            states[l] = 0;
+            for (int w = 0; w < kWaves; ++w) states[l] += wst[l][w];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ void scan_groups(uint32_t* gcnt, const unsigned long long* gstates, uint32_t ng,
+                            uint32_t& total, unsigned long long& states) {
+    scan_groups_n<1>(&gcnt, &gstates, &ng, &total, &states);
 }
 
 // Ends a level: scans the group counts of both producers into list offsets,
@@ -1293,10 +1327,14 @@ __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
     const uint32_t ng0 = (a.nbatch + kGroupWords - 1) / kGroupWords;
     const uint32_t nw1 = (ctl->cnt0[cur] + 63) / 64;
     const uint32_t ng1 = (nw1 + kGroupWords - 1) / kGroupWords;
-    uint32_t t0 = 0, t1 = 0;
-    unsigned long long s0 = 0, s1 = 0;
-    scan_groups(a.c0.gcnt, a.c0.gstates, ng0, t0, s0);
-    scan_groups(a.c1.gcnt, a.c1.gstates, ng1, t1, s1);
+    uint32_t* gc[2] = {a.c0.gcnt, a.c1.gcnt};
+    const unsigned long long* gs[2] = {a.c0.gstates, a.c1.gstates};
+    const uint32_t ng[2] = {ng0, ng1};
+    uint32_t tt[2] = {0, 0};
+    unsigned long long ss[2] = {0, 0};
+    scan_groups_n<2>(gc, gs, ng, tt, ss);
+    const uint32_t t0 = tt[0], t1 = tt[1];
+    const unsigned long long s0 = ss[0], s1 = ss[1];
     if (threadIdx.x == 0) {
         ctl->lvl[d + 1][0] = s0;
         ctl->lvl[d + 1][1] = s1;
