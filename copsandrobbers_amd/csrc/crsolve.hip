@@ -34,6 +34,7 @@
 //   k_cop_finalize  nx0 & ~won0 becomes the cop-turn states of depth d+1.
 // States whose depth would reach 127 stay unmarked (the 7-bit cap, :189,
 // :220, :280), so levels run d = 0..125.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1980,6 +1981,7 @@ struct cr_solver {
     std::vector<std::string> korder;
     std::vector<hipEvent_t> events;  // reusable timing events
     std::string time_only;           // when set, only launches of this kernel are timed
+    const Timer* launch_timer = nullptr;  // events the next single-kernel launch records itself
 
     ~cr_solver() { release(); }
 
@@ -2100,18 +2102,21 @@ struct Timer {
     const char* name;
     hipEvent_t a = nullptr, b = nullptr;
     bool on = false;
-    Timer(cr_solver* s_, const char* n) : s(s_), name(n) {
+    bool in_launch = false;  // a and b are set by the one kernel launch timed (hipExtLaunchKernelGGL)
+    Timer(cr_solver* s_, const char* n, bool one_launch = false) : s(s_), name(n), in_launch(one_launch) {
         on = s->timing && (s->time_only.empty() || s->time_only == n);
         if (on) {
             a = acquire_event(s);
             b = acquire_event(s);
-            hipEventRecord(a, s->stream);
+            if (!in_launch) hipEventRecord(a, s->stream);
         }
     }
     void stop() {
-        if (!on) return;
+        if (!on || in_launch) return;
         hipEventRecord(b, s->stream);
     }
+    hipEvent_t start_event() const { return on ? a : nullptr; }
+    hipEvent_t stop_event() const { return on ? b : nullptr; }
     // called after the stream has been synchronised
     void collect() {
         if (!on || a == nullptr) return;
@@ -2131,8 +2136,10 @@ struct Timer {
 };
 
 // A timer for the launch about to be enqueued, kept until the chunk syncs.
-Timer* start_timer(cr_solver* s, const char* name) {
-    s->timers->emplace_back(new Timer(s, name));
+// one_launch: the timed region is a single kernel launch that records the
+// timer's events itself (no event packets between launches)
+Timer* start_timer(cr_solver* s, const char* name, bool one_launch = false) {
+    s->timers->emplace_back(new Timer(s, name, one_launch));
     return s->timers->back().get();
 }
 
@@ -2165,14 +2172,17 @@ struct LaunchInit {
 template <int K>
 struct LaunchPush {
     static int run(cr_solver* s, LevelArgs a) {
+        const Timer* t = s->launch_timer;
+        hipEvent_t e0 = t ? t->start_event() : nullptr, e1 = t ? t->stop_event() : nullptr;
         if (s->d_csr) {
-            hipLaunchKernelGGL(k_cop_push_csr, dim3(s->push_blocks), dim3(kBlock), 0, s->stream, s->gv, a, s->d_heads,
-                               s->d_csr);
+            hipExtLaunchKernelGGL(k_cop_push_csr, dim3(s->push_blocks), dim3(kBlock), 0, s->stream, e0, e1, 0u,
+                                  s->gv, a, static_cast<const unsigned long long*>(s->d_heads),
+                                  static_cast<const uint32_t*>(s->d_csr));
             return 0;
         }
         if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_cop_push<K>), s->gv.image_bytes)) return rc;
-        hipLaunchKernelGGL(k_cop_push<K>, dim3(s->push_blocks), dim3(kBlock), s->gv.image_bytes, s->stream,
-                           s->gv, a);
+        hipExtLaunchKernelGGL(k_cop_push<K>, dim3(s->push_blocks), dim3(kBlock), s->gv.image_bytes, s->stream, e0, e1,
+                              0u, s->gv, a);
         return 0;
     }
 };
@@ -2896,9 +2906,11 @@ int do_loop(cr_solver* s) {
         }
         if (!pulls) la.pull_rows = 0xffffffffu;
         for (int l = 0; l < s->levels_per_sync; ++l) {
-            Timer* t = start_timer(s, "cop_push");
-            if ((rc = dispatch_k<LaunchPush>(s->k, s, la)) != 0) return rc;
-            t->stop();
+            Timer* t = start_timer(s, "cop_push", true);
+            s->launch_timer = t;
+            rc = dispatch_k<LaunchPush>(s->k, s, la);
+            s->launch_timer = nullptr;
+            if (rc) return rc;
             if (pulls && (rc = dispatch_k<LaunchPull>(s->k, s, la)) != 0) return rc;
             t = start_timer(s, "robber_step");
             if ((rc = dispatch_w<LaunchRobber>(s->W, s, la)) != 0) return rc;
