@@ -129,6 +129,7 @@ struct LevelArgs {
     uint64_t* log_bits;        // depth log: W robber words of each entry
     int32_t n, W;
     int32_t fuse;              // pulled levels: the last pull stage does k_cop_finalize's work
+    uint32_t open_factor;      // robber step: rows with open <= open_factor * new cop-turn states take the open-state path
 };
 
 // Work items [0, n) in 8 contiguous segments, one per XCD (workgroups are
@@ -1074,6 +1075,12 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
     const uint32_t wid = wave_id();
     const int n = gv.n;
     const uint8_t nb = won_byte(level + 1);
+    uint64_t full[W];  // the row's robber positions
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        const int bits = min(64, max(0, n - 64 * q));
+        full[q] = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+    }
     for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
         unsigned long long rows = 0, found = 0;
         for (int j = 0; j < kWordsPerWave; ++j) {
@@ -1083,12 +1090,13 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
             const bool live = i < count;
             const uint32_t row = live ? list[i] : 0u;
             const uint64_t o = static_cast<uint64_t>(row) * W;
-            uint64_t w0[W], w1[W], f0[W], nw[W], cand[W];
+            uint64_t w0[W], w1[W], f0[W], nw[W], open[W];
             if (live) {  // 16-byte row loads (rows are 16-byte aligned for even W)
                 load_row<W>(a.won0 + o, w0);
                 load_row<W>(a.won1 + o, w1);
                 load_row<W>(a.fr0 + o, f0);
             }
+            uint32_t nopen = 0, nnew = 0;
 #pragma unroll
             for (int q = 0; q < W; ++q) {
                 if (!live) {
@@ -1097,31 +1105,58 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
                     f0[q] = 0;
                 }
                 nw[q] = 0;
-                cand[q] = 0;
+                open[q] = ~w0[q] & full[q];
+                nopen += __popcll(open[q]);
+                nnew += __popcll(f0[q]);
             }
-            // Only a robber with a move onto a cop-turn state won at this depth
-            // can change: its other moves were already counted at earlier depths.
+            if (nopen <= a.open_factor * nnew) {
+                // Few cop-turn states of the row are still open: a robber with a
+                // move onto an open state stays open, and every other robber not
+                // yet won is won at this depth (had all its moves been won by an
+                // earlier depth, an earlier level would have taken it).
+                uint64_t stay[W];
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                uint64_t x = f0[w];
-                while (x) {
-                    const uint64_t* im = lim + (64 * w + __builtin_ctzll(x)) * W;
-                    x &= x - 1;
+                for (int q = 0; q < W; ++q) stay[q] = 0;
 #pragma unroll
-                    for (int q = 0; q < W; ++q) cand[q] |= im[q];
+                for (int w = 0; w < W; ++w) {
+                    uint64_t x = open[w];
+                    while (x) {
+                        const uint64_t* im = lim + (64 * w + __builtin_ctzll(x)) * W;
+                        x &= x - 1;
+#pragma unroll
+                        for (int q = 0; q < W; ++q) stay[q] |= im[q];
+                    }
                 }
-            }
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                uint64_t x = cand[w] & ~w1[w];
-                while (x) {
-                    const int rb = __builtin_ctzll(x);
-                    x &= x - 1;
-                    const uint64_t* om = lom + (64 * w + rb) * W;
-                    uint64_t bad = 0;
+                for (int q = 0; q < W; ++q) nw[q] = ~stay[q] & ~w1[q] & full[q];
+            } else {
+                // Only a robber with a move onto a cop-turn state won at this depth
+                // can change: its other moves were already counted at earlier depths.
+                uint64_t cand[W];
 #pragma unroll
-                    for (int q = 0; q < W; ++q) bad |= om[q] & ~w0[q];
-                    nw[w] |= static_cast<uint64_t>(bad == 0) << rb;
+                for (int q = 0; q < W; ++q) cand[q] = 0;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    uint64_t x = f0[w];
+                    while (x) {
+                        const uint64_t* im = lim + (64 * w + __builtin_ctzll(x)) * W;
+                        x &= x - 1;
+#pragma unroll
+                        for (int q = 0; q < W; ++q) cand[q] |= im[q];
+                    }
+                }
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    uint64_t x = cand[w] & ~w1[w];
+                    while (x) {
+                        const int rb = __builtin_ctzll(x);
+                        x &= x - 1;
+                        const uint64_t* om = lom + (64 * w + rb) * W;
+                        uint64_t bad = 0;
+#pragma unroll
+                        for (int q = 0; q < W; ++q) bad |= om[q] & ~w0[q];
+                        nw[w] |= static_cast<uint64_t>(bad == 0) << rb;
+                    }
                 }
             }
             uint32_t pop = 0;
@@ -1947,6 +1982,7 @@ struct cr_solver {
     uint32_t tile_blocks = 8192;              // k_pull_tile: workgroups
     uint32_t tile_chunk = 16;                 // k_pull_tile: consecutive tiles a workgroup takes at a time
     bool tile1 = true;                        // k_pull_tile1 runs stage 1 of a tiled multi-cop pull
+    uint32_t open_factor = 2;                 // robber step: open-state path where open <= factor * new
     uint32_t pull_gate = 16;                  // a chunk may pull if its first frontier * pull_gate > pull_rows
     bool fuse_finalize = true;                // tuning: with the depth log, pulled levels finalize in the last stage
     bool fuse_ok = false;                     // the last pull stage runs as LDS tiles (image and one row per vertex fit)
@@ -2345,6 +2381,7 @@ LevelArgs level_args(cr_solver* s) {
     a.in_mask = s->d_inmask;
     a.fr0 = s->d_fr0;
     a.filter = s->filter;
+    a.open_factor = s->open_factor;
     return a;
 }
 
@@ -3456,6 +3493,10 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             return 0;
         case CR_TUNE_FUSE_FINALIZE: s->fuse_finalize = value != 0; return 0;
         case CR_TUNE_PULL_TILE1: s->tile1 = value != 0; return 0;
+        case CR_TUNE_ROBBER_OPEN:
+            if (value < 0 || value > 1000) return fail(CR_ERR_ARG, "robber open factor %lld", static_cast<long long>(value));
+            s->open_factor = static_cast<uint32_t>(value);
+            return 0;
         case CR_TUNE_PULL_GATE:
             if (value < 0 || value > (1 << 20)) return fail(CR_ERR_ARG, "pull gate %lld", static_cast<long long>(value));
             s->pull_gate = static_cast<uint32_t>(value);

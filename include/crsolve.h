@@ -246,9 +246,12 @@ enum cr_tuning {
                                     stage settles the new cop-turn states itself (no finalize pass) */
     CR_TUNE_PULL_TILE1 = 14,     /* 1 (default): a tiled pull's first stage takes CR_TUNE_PULL_TILE_ROWS
                                     consecutive cop multisets per tile (else one per tile) */
-    CR_TUNE_PULL_GATE = 15       /* levels enqueued between host synchronisations may pull only if the
+    CR_TUNE_PULL_GATE = 15,      /* levels enqueued between host synchronisations may pull only if the
                                     first one's robber frontier is above 1/gate of the pull threshold
                                     (else they push and launch no pull kernels); 0 = always (default 16) */
+    CR_TUNE_ROBBER_OPEN = 16     /* the robber step finds a row's new robber-turn states from its open
+                                    cop-turn states where open <= value * new cop-turn states, else from
+                                    the new ones; 0..1000 (default 2) */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:
