@@ -354,17 +354,20 @@ template <int K>
 __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     if (a.ctl->done || a.ctl->pull) return;  // uniform: before any barrier
-    load_image(lds, gv.image, gv.image_bytes);
     const uint32_t cur = a.ctl->level & 1;
+    const uint32_t count = a.ctl->cnt1[cur];
+    const XcdSplit sp = xcd_split(count);
+    // a workgroup whose waves have no frontier row skips the image copy (the
+    // grid is sized for the largest frontier; small levels use few workgroups)
+    if (sp.first - wave_id() >= sp.end) return;  // uniform: the workgroup's first wave
+    load_image(lds, gv.image, gv.image_bytes);
     const uint32_t* rank_tab = reinterpret_cast<const uint32_t*>(lds);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const uint16_t* in_off = reinterpret_cast<const uint16_t*>(lds + gv.off_in_off);
     const uint8_t* in_nbr = lds + gv.off_in_nbr;
     const int lane = threadIdx.x & 63;
     const int W = gv.W;
-    const uint32_t count = a.ctl->cnt1[cur];
     const uint32_t* list = a.list1[cur];
-    const XcdSplit sp = xcd_split(count);
     for (uint32_t it = sp.first; it < sp.end; it += sp.stride) {
         const uint32_t row = list[it];
         uint64_t m[4] = {0, 0, 0, 0};
@@ -1065,13 +1068,14 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
     __shared__ uint64_t lds_im[CR_MAX_N * W];
     const int lane = threadIdx.x & 63;
     if (a.ctl->done) return;
-    const uint64_t* lom = stage_out_masks<W>(lds_om, a.out_mask, gv.n);
-    const uint64_t* lim = stage_out_masks<W>(lds_im, a.in_mask, gv.n);
     const uint32_t level = a.ctl->level, cur = level & 1;
     const uint32_t count = a.ctl->cnt0[cur];
     const uint32_t* list = a.list0[cur];
     const uint32_t nwords = (count + 63) / 64;
     const uint32_t ngroups = (nwords + kGroupWords - 1) / kGroupWords;
+    if (blockIdx.x >= ngroups) return;  // no rows here: skip the mask staging (uniform)
+    const uint64_t* lom = stage_out_masks<W>(lds_om, a.out_mask, gv.n);
+    const uint64_t* lim = stage_out_masks<W>(lds_im, a.in_mask, gv.n);
     const uint32_t wid = wave_id();
     const int n = gv.n;
     const uint8_t nb = won_byte(level + 1);
