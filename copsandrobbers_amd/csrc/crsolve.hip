@@ -687,6 +687,7 @@ constexpr int kTileBlock = 256;  // threads of a tile workgroup
 constexpr int kTileLoads = 16;   // input words a thread stages per tile: n * r * W <= kTileLoads * kTileBlock
 constexpr uint32_t kLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
 
+
 template <int I, int J, int W>
 __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArgs a, SepStage st) {
     extern __shared__ __align__(16) uint8_t lds[];
@@ -926,9 +927,15 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArg
 // r * n consecutive rows of H_1. Thread x always loads word x mod W of B_t,
 // t = (x mod rW) / W, for rows v = x / rW + m * (blockDim / rW), so each
 // thread keeps one B_t and its rank terms. Same chunked walk and register
-// staging as k_pull_tile.
+// staging as k_pull_tile, with kTile1Block threads: a run of r rows starts at
+// any 32-byte offset, so longer runs waste less of the 64-byte sectors they
+// touch, and their loads need the larger workgroup. At W = 4, r = 9 (a 288-byte
+// LDS row) measured best: r = 8 gives 256-byte LDS rows, which start on the
+// same banks, and the lanes gather rows of random neighbours.
+constexpr int kTile1Block = 512;
+
 template <int J, int W>
-__global__ __launch_bounds__(kTileBlock) void k_pull_tile1(GraphView gv, LevelArgs a, SepStage st) {
+__global__ __launch_bounds__(kTile1Block) void k_pull_tile1(GraphView gv, LevelArgs a, SepStage st) {
     static_assert(J > 0, "stage 1 of a single-cop pull is its last stage");
     extern __shared__ __align__(16) uint8_t lds[];
     if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
@@ -939,7 +946,7 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile1(GraphView gv, LevelAr
     const uint32_t n = gv.n;
     const uint32_t Mp = gv.M + 1;
     const uint32_t r = st.r, rW = r * W;
-    const uint32_t per = kTileBlock / rW;  // input rows v that one pass of the workgroup loads
+    const uint32_t per = kTile1Block / rW;  // input rows v that one pass of the workgroup loads
     const bool loader = threadIdx.x < per * rW;
     const uint32_t tq = threadIdx.x % rW, my_t = tq / W, my_w = tq % W, v0 = threadIdx.x / rW;
     uint64_t* buf = reinterpret_cast<uint64_t*>(lds + ((gv.image_bytes + 15) & ~15u));
@@ -1985,7 +1992,7 @@ struct cr_solver {
     uint32_t tile_rows = 4;                   // k_pull_tile: rows of each input block per tile
     uint32_t tile_blocks = 8192;              // k_pull_tile: workgroups
     uint32_t tile_chunk = 16;                 // k_pull_tile: consecutive tiles a workgroup takes at a time
-    bool tile1 = true;                        // k_pull_tile1 runs stage 1 of a tiled multi-cop pull
+    uint32_t tile1_rows = 9;                  // k_pull_tile1 (stage 1 of a tiled multi-cop pull): B per tile; 0 = off
     uint32_t open_factor = 2;                 // robber step: open-state path where open <= factor * new
     uint32_t pull_gate = 16;                  // a chunk may pull if its first frontier * pull_gate > pull_rows
     bool fuse_finalize = true;                // tuning: with the depth log, pulled levels finalize in the last stage
@@ -2246,15 +2253,17 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
     } else {
         SepStage st{};
         // k_pull_tile1 takes stage 1 of a multi-cop pull (r consecutive B per tile)
-        const bool tile1 = I == 1 && K > 1 && s->tile1;
-        st.r = I == 1 && !tile1 ? 1u : s->tile_rows;
+        const bool tile1 = I == 1 && K > 1 && s->tile1_rows > 0;
+        st.r = tile1 ? s->tile1_rows : I == 1 ? 1u : s->tile_rows;
         // LDS of a tile: image and n * r rows; fewer rows per tile (or the row
         // kernel) where a large graph image leaves too little room
         auto tile_lds = [&](uint32_t r) { return ((s->gv.image_bytes + 15) & ~15u) + s->g.n * r * W * 8; };
         auto fits = [&](uint32_t r) {
-            if (tile_lds(r) > kLdsBytes || r * W > static_cast<uint32_t>(kTileBlock)) return false;
+            if (tile_lds(r) > kLdsBytes) return false;
             if (tile1)  // each thread loads kTileLoads rows v of one B_t
-                return s->g.n <= static_cast<uint32_t>(kTileLoads) * (kTileBlock / (r * W));
+                return r * W <= static_cast<uint32_t>(kTile1Block) &&
+                       s->g.n <= static_cast<uint32_t>(kTileLoads) * (kTile1Block / (r * W));
+            if (r * W > static_cast<uint32_t>(kTileBlock)) return false;
             return s->g.n * r * W <= static_cast<uint32_t>(kTileLoads * kTileBlock);
         };
         while (st.r > 1 && !fits(st.r)) --st.r;
@@ -2289,7 +2298,8 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
         if (int rc = raise_lds_limit(fn, lds)) return rc;
         blocks = std::max<uint64_t>(kXcds, (blocks + kXcds - 1) / kXcds * kXcds);
         Timer* t = start_timer(s, "sep_stage");
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(tiles ? kTileBlock : kBlock), lds, s->stream, s->gv, a, st);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(!tiles ? kBlock : tile1 ? kTile1Block : kTileBlock), lds, s->stream,
+                           s->gv, a, st);
         t->stop();
         return launch_pull_stage<K, W, I + 1>(s, a, st.out);
     }
@@ -3496,7 +3506,10 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             s->tile_blocks = static_cast<uint32_t>(value);
             return 0;
         case CR_TUNE_FUSE_FINALIZE: s->fuse_finalize = value != 0; return 0;
-        case CR_TUNE_PULL_TILE1: s->tile1 = value != 0; return 0;
+        case CR_TUNE_PULL_TILE1:
+            if (value < 0 || value > kTileMaxRows) return fail(CR_ERR_ARG, "stage-1 tile rows %lld", static_cast<long long>(value));
+            s->tile1_rows = static_cast<uint32_t>(value);
+            return 0;
         case CR_TUNE_ROBBER_OPEN:
             if (value < 0 || value > 1000) return fail(CR_ERR_ARG, "robber open factor %lld", static_cast<long long>(value));
             s->open_factor = static_cast<uint32_t>(value);

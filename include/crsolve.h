@@ -244,8 +244,9 @@ enum cr_tuning {
     CR_TUNE_PULL_TILE_CHUNK = 12, /* consecutive tiles a pull workgroup takes at a time (default 16) */
     CR_TUNE_FUSE_FINALIZE = 13,  /* 1 (default): where the depth log is kept, a pulled level's last
                                     stage settles the new cop-turn states itself (no finalize pass) */
-    CR_TUNE_PULL_TILE1 = 14,     /* 1 (default): a tiled pull's first stage takes CR_TUNE_PULL_TILE_ROWS
-                                    consecutive cop multisets per tile (else one per tile) */
+    CR_TUNE_PULL_TILE1 = 14,     /* consecutive cop multisets per tile of a tiled pull's first stage,
+                                    1..16, fewer where the tile does not fit (default 9); 0 = the first
+                                    stage runs k_pull_tile */
     CR_TUNE_PULL_GATE = 15,      /* levels enqueued between host synchronisations may pull only if the
                                     first one's robber frontier is above 1/gate of the pull threshold
                                     (else they push and launch no pull kernels); 0 = always (default 16) */
