@@ -1457,10 +1457,11 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
     const uint32_t prev = nxt ^ 1;
     const uint32_t nw0 = a.nbatch;
     const uint32_t nw1 = init_mode ? a.nbatch : (ctl->cnt0[prev] + 63) / 64;
-    const uint32_t span0 = (nw0 + kGroupWords - 1) / kGroupWords * kGroupWords;  // space 0 words, padded
+    const uint32_t span0 = (nw0 + 63) / 64 * 64;  // space 0 words, padded to whole waves
+    const bool small = a.log_rows == nullptr;     // no depth log: a table that stays in the caches
     const uint32_t lane = threadIdx.x & 63;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t - lane < span0 + nw1; t += gridDim.x * blockDim.x) {
-        const bool space1 = t >= span0;
+        const bool space1 = t >= span0;  // uniform per wave
         const uint32_t word = space1 ? t - span0 : t;
         const Compact& c = space1 ? a.c1 : a.c0;
         const uint32_t nw = space1 ? nw1 : nw0;
@@ -1471,14 +1472,39 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
             const uint32_t v = __shfl_up(incl, off, kGroupWords);
             if (static_cast<int>(word % kGroupWords) >= off) incl += v;
         }
-        if (bits == 0) continue;
-        uint32_t pos = c.gcnt[word / kGroupWords] + incl - pc;
+        const uint32_t pos = bits ? c.gcnt[word / kGroupWords] + incl - pc : 0u;
         uint32_t* dst = space1 ? a.list1[nxt] : a.list0[nxt];
-        uint64_t rest = bits;
-        while (rest) {
-            const uint32_t idx = word * 64 + static_cast<uint32_t>(__builtin_ctzll(rest));
-            rest &= rest - 1;
-            dst[pos++] = (space1 && !init_mode) ? a.list0[prev][idx] : idx;
+        unsigned long long live = __ballot(bits != 0);
+        if (!live) continue;  // uniform
+        uint32_t maxpc = pc;  // the wave's densest word
+        for (int off = 32; off > 0; off >>= 1) maxpc = max(maxpc, static_cast<uint32_t>(__shfl_xor(static_cast<int>(maxpc), off)));
+        if (small || 4 * maxpc <= static_cast<uint32_t>(__popcll(live))) {
+            // lists that stay in the caches (the depth log is kept from 2^24
+            // configurations on), or sparse words (in HBM a scattered store
+            // costs several contiguous ones): each lane writes its own word's
+            // entries
+            uint32_t p = pos;
+            uint64_t rest = bits;
+            while (rest) {
+                const uint32_t idx = word * 64 + static_cast<uint32_t>(__builtin_ctzll(rest));
+                rest &= rest - 1;
+                dst[p++] = (space1 && !init_mode) ? a.list0[prev][idx] : idx;
+            }
+            continue;
+        }
+        // dense words: the wave writes them one at a time, lane j the entry of
+        // bit j, so stores (and the list0 reads of space 1) are contiguous
+        while (live) {
+            const int w = __builtin_ctzll(live);
+            live &= live - 1;
+            const uint32_t lo = __shfl(static_cast<uint32_t>(bits), w);
+            const uint32_t hi = __shfl(static_cast<uint32_t>(bits >> 32), w);
+            const uint32_t wpos = __shfl(pos, w);
+            const uint64_t wb = (static_cast<uint64_t>(hi) << 32) | lo;
+            if ((wb >> lane) & 1ull) {
+                const uint32_t idx = (word - lane + w) * 64 + lane;
+                dst[wpos + __popcll(wb & ((1ull << lane) - 1))] = (space1 && !init_mode) ? a.list0[prev][idx] : idx;
+            }
         }
     }
 }
