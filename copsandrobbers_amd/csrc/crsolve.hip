@@ -1584,13 +1584,13 @@ struct LogList {
     uint32_t turn_byte;        // turn << 8 | depth byte
 };
 
-// Per chunk c and list L: cnt[c * nl + L] entries of L have rows in chunk c,
-// the first at beg[c * nl + L] (cnt is zeroed before; beg is only read where
-// cnt > 0). One thread per entry; the first entry of a chunk finds the chunk's
+// Per chunk c and list L: cnt[L * nchunks + c] entries of L have rows in
+// chunk c, the first at beg[L * nchunks + c] (cnt is zeroed before; beg is
+// only read where cnt > 0). One thread per entry; the first entry of a chunk finds the chunk's
 // last one by binary search over at most R entries.
 __global__ __launch_bounds__(kBlock) void k_log_chunks(const LogList* lists, int32_t nl, uint64_t total,
                                                        const uint32_t* rows, uint32_t* beg, uint32_t* cnt,
-                                                       uint32_t R) {
+                                                       uint32_t R, uint32_t nchunks) {
     __shared__ LogList sl[256];
     for (int L = threadIdx.x; L < nl; L += blockDim.x) sl[L] = lists[L];
     __syncthreads();
@@ -1611,8 +1611,9 @@ __global__ __launch_bounds__(kBlock) void k_log_chunks(const LogList* lists, int
             const uint32_t mid = (a + b) >> 1;
             if (lr[mid] / R == c) a = mid + 1; else b = mid;
         }
-        beg[static_cast<uint64_t>(c) * nl + lo] = i;
-        cnt[static_cast<uint64_t>(c) * nl + lo] = a - i;
+        // list-major: consecutive chunk-first entries of a list store side by side
+        beg[static_cast<uint64_t>(lo) * nchunks + c] = i;
+        cnt[static_cast<uint64_t>(lo) * nchunks + c] = a - i;
     }
 }
 
@@ -1663,8 +1664,8 @@ __global__ __launch_bounds__(kBlock) void k_apply_log(const LogList* lists, int3
         __syncthreads();  // the previous chunk is written back
         uint32_t c = 0;
         if (threadIdx.x < static_cast<uint32_t>(nl)) {
-            c = cnt[static_cast<uint64_t>(ch) * nl + threadIdx.x];
-            if (c) s_beg[threadIdx.x] = beg[static_cast<uint64_t>(ch) * nl + threadIdx.x];
+            c = cnt[static_cast<uint64_t>(threadIdx.x) * nchunks + ch];
+            if (c) s_beg[threadIdx.x] = beg[static_cast<uint64_t>(threadIdx.x) * nchunks + ch];
         }
         uint32_t total = 0;
         const uint32_t pre = block_scan_u32(c, total);
@@ -2608,7 +2609,7 @@ struct LaunchApplyLog {
                            dim3(kBlock), 0, s->stream, cnt, cells, 0u);
         const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((total + kBlock - 1) / kBlock, 8192)));
         hipLaunchKernelGGL(k_log_chunks, dim3(sgrid), dim3(kBlock), 0, s->stream, s->d_lists, nl, total, s->d_log_rows,
-                           beg, cnt, R);
+                           beg, cnt, R, nchunks);
         const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nchunks, 8192));
         if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_apply_log<W>), lds)) return rc;
         hipLaunchKernelGGL(k_apply_log<W>, dim3(grid), dim3(kBlock), lds, s->stream, s->d_lists, nl, s->d_log_rows,
