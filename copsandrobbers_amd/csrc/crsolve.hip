@@ -2027,7 +2027,8 @@ struct cr_solver {
     bool stage2_cleared = false;              // the pull being enqueued clears the frontier in stage 2
     uint32_t pull_rows = 0xffffffffu;         // frontier rows above which a level pulls
     uint32_t push_blocks = 2048;
-    uint32_t row_blocks = 2048;
+    uint32_t row_blocks = 32768;
+    uint32_t list_blocks = 4096;
     int32_t levels_per_sync = 8;
     // limited-visibility variant (macro.hip)
     crd::MacroPlan* macro = nullptr;
@@ -2043,8 +2044,11 @@ struct cr_solver {
     uint32_t row_grid() const {
         return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(ngroups(), row_blocks)));
     }
-    uint32_t writer_grid() const {
+    uint32_t writer_grid() const {  // k_write_lists: a thread per flag word of both lists
         return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((2 * nbatch() + 255) / 256 + 1, 4096)));
+    }
+    uint32_t log_grid() const {  // k_log_lists: a thread per list entry (up to 2C), grid-stride
+        return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((2 * C + 255) / 256, list_blocks)));
     }
     // results
     std::vector<uint64_t> pass_counts;
@@ -2621,7 +2625,7 @@ struct LaunchApplyLog {
 template <int W>
 struct LaunchLogLists {
     static int run(cr_solver* s, LevelArgs a) {
-        hipLaunchKernelGGL(k_log_lists<W>, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, a);
+        hipLaunchKernelGGL(k_log_lists<W>, dim3(s->log_grid()), dim3(kBlock), 0, s->stream, a);
         return 0;
     }
 };
@@ -3511,6 +3515,10 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
         case CR_TUNE_ROW_BLOCKS:
             if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "row blocks %lld", static_cast<long long>(value));
             s->row_blocks = static_cast<uint32_t>(value);
+            return 0;
+        case CR_TUNE_LOG_BLOCKS:
+            if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "log blocks %lld", static_cast<long long>(value));
+            s->list_blocks = static_cast<uint32_t>(value);
             return 0;
         case CR_TUNE_PULL_SCALE:
             if (value < 0) value = -1;  // the size-dependent default

@@ -225,7 +225,7 @@ int cr_count_transitions(cr_solver* s, uint64_t* total);
 enum cr_tuning {
     CR_TUNE_PUSH_FILTER = 1,    /* 1: cop push skips targets already won (default 1) */
     CR_TUNE_PUSH_BLOCKS = 2,    /* workgroups of the cop-push kernel (default 2048) */
-    CR_TUNE_ROW_BLOCKS = 3,     /* workgroups of the row kernels (default 2048) */
+    CR_TUNE_ROW_BLOCKS = 3,     /* workgroups of the row kernels (default 32768, at most one per group) */
     CR_TUNE_LEVELS_PER_SYNC = 4, /* depth levels enqueued per host synchronisation (default 8) */
     CR_TUNE_PRED_CSR = 5,        /* 1: cr_build_aux_graph precomputes predecessor lists (default 0) */
     CR_TUNE_PULL_SCALE = 6,      /* a level gathers (factorised pull) instead of pushing when its robber
@@ -250,9 +250,11 @@ enum cr_tuning {
     CR_TUNE_PULL_GATE = 15,      /* levels enqueued between host synchronisations may pull only if the
                                     first one's robber frontier is above 1/gate of the pull threshold
                                     (else they push and launch no pull kernels); 0 = always (default 16) */
-    CR_TUNE_ROBBER_OPEN = 16     /* the robber step finds a row's new robber-turn states from its open
+    CR_TUNE_ROBBER_OPEN = 16,    /* the robber step finds a row's new robber-turn states from its open
                                     cop-turn states where open <= value * new cop-turn states, else from
                                     the new ones; 0..1000 (default 2) */
+    CR_TUNE_LOG_BLOCKS = 17      /* workgroups of k_log_lists, which appends the level lists to the
+                                    depth log (default 4096) */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:
