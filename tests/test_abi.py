@@ -32,6 +32,14 @@ def test_library_loads_and_exports_declared_symbols():
     assert set(syms) == set(cra.EXPORTS)
 
 
+def test_tuning_keys_match_header():
+    """The Python tuning names map onto exactly the CR_TUNE_* values of the header."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    keys = {name.lower(): int(v) for name, v in re.findall(r"\bCR_TUNE_(\w+)\s*=\s*(\d+)", text)}
+    assert len(keys) >= 17
+    assert keys == cra.TUNING
+
+
 def test_exported_symbols_are_c_linkage():
     out = subprocess.run(["nm", "-D", "--defined-only", cra.library_path()], capture_output=True,
                          text=True, check=True).stdout
