@@ -2614,7 +2614,7 @@ struct LaunchApplyLog {
         const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((total + kBlock - 1) / kBlock, 8192)));
         hipLaunchKernelGGL(k_log_chunks, dim3(sgrid), dim3(kBlock), 0, s->stream, s->d_lists, nl, total, s->d_log_rows,
                            beg, cnt, R, nchunks);
-        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nchunks, 8192));
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nchunks, 65536));
         if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_apply_log<W>), lds)) return rc;
         hipLaunchKernelGGL(k_apply_log<W>, dim3(grid), dim3(kBlock), lds, s->stream, s->d_lists, nl, s->d_log_rows,
                            s->d_log_bits, beg, cnt, s->d_table, n, static_cast<uint32_t>(s->C), R);
