@@ -2321,7 +2321,9 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
             lds = tile_lds(st.r);
             const uint64_t ntiles = tile1 ? (st.cnt_b + st.r - 1) / st.r
                                           : static_cast<uint64_t>(st.cnt_b) * ((st.cnt_a_in + st.r - 1) / st.r);
-            st.chunk = s->tile_chunk;
+            // chunks of up to tile_chunk tiles, but at least ~2048 chunks, so that
+            // small stages still spread over every CU (ER k = 3: 2-5 tiles)
+            st.chunk = static_cast<uint32_t>(std::min<uint64_t>(s->tile_chunk, std::max<uint64_t>(1, (ntiles + 2047) / 2048)));
             blocks = std::min<uint64_t>((ntiles + st.chunk - 1) / st.chunk, s->tile_blocks);
         } else {
             blocks = std::min<uint64_t>(((st.rows + 63) / 64 + kWavesPerBlock - 1) / kWavesPerBlock, 16384);

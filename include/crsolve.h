@@ -241,7 +241,8 @@ enum cr_tuning {
                                     each depth's new states are logged and written to the table in row
                                     chunks. Takes effect at the next cr_build_aux_graph */
     CR_TUNE_DEPTH_LOG_MIN = 11,  /* configurations below which no depth log is kept (default 2^24) */
-    CR_TUNE_PULL_TILE_CHUNK = 12, /* consecutive tiles a pull workgroup takes at a time (default 16) */
+    CR_TUNE_PULL_TILE_CHUNK = 12, /* consecutive tiles a pull workgroup takes at a time, at most (default 16;
+                                    fewer where a stage has under 2048 * 16 tiles) */
     CR_TUNE_FUSE_FINALIZE = 13,  /* 1 (default): where the depth log is kept, a pulled level's last
                                     stage settles the new cop-turn states itself (no finalize pass) */
     CR_TUNE_PULL_TILE1 = 14,     /* consecutive cop multisets per tile of a tiled pull's first stage,
