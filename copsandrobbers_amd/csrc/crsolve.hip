@@ -2021,7 +2021,7 @@ struct cr_solver {
     uint32_t tile_chunk = 16;                 // k_pull_tile: consecutive tiles a workgroup takes at a time
     uint32_t tile1_rows = 9;                  // k_pull_tile1 (stage 1 of a tiled multi-cop pull): B per tile; 0 = off
     uint32_t open_factor = 2;                 // robber step: open-state path where open <= factor * new
-    uint32_t pull_gate = 16;                  // a chunk may pull if its first frontier * pull_gate > pull_rows
+    uint32_t pull_gate = 4;                   // a chunk may pull if its first frontier * pull_gate > pull_rows
     bool fuse_finalize = true;                // tuning: with the depth log, pulled levels finalize in the last stage
     bool fuse_ok = false;                     // the last pull stage runs as LDS tiles (image and one row per vertex fit)
     bool stage2_cleared = false;              // the pull being enqueued clears the frontier in stage 2
@@ -2537,9 +2537,12 @@ void set_pull_threshold(cr_solver* s) {
     const double pred = std::pow(std::max(1.0, static_cast<double>(edges) / n), K);
     // Measured on MI355X: a push costs ~4-6 ns per frontier row while the
     // won0/nx0 bitsets fit in the 256 MiB last-level cache and ~10 ns beyond,
-    // a pull ~0.007 ns per row read; the default scale follows that split.
+    // a pull ~0.007 ns per row read. The defaults are the measured optima of
+    // the scale on ER(200, 0.03) k = 3 (cached: 150-160) and planar(200) k = 4
+    // (uncached: 150-190); pushes get cheaper per row as won0 fills, which
+    // this row count does not model.
     const bool cached = 2ull * s->C * s->W * 8 <= (256ull << 20);
-    const int64_t scale = s->pull_scale >= 0 ? s->pull_scale : (cached ? 500 : 190);
+    const int64_t scale = s->pull_scale >= 0 ? s->pull_scale : (cached ? 150 : 190);
     const double rows = pull_reads / pred * static_cast<double>(scale) / 1000.0;
     s->pull_rows = rows >= 4.0e9 ? 0xffffffffu : static_cast<uint32_t>(rows);
 }

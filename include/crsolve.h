@@ -230,7 +230,7 @@ enum cr_tuning {
     CR_TUNE_PRED_CSR = 5,        /* 1: cr_build_aux_graph precomputes predecessor lists (default 0) */
     CR_TUNE_PULL_SCALE = 6,      /* a level gathers (factorised pull) instead of pushing when its robber
                                     frontier exceeds value/1000 times the estimated break-even row count;
-                                    0: always pull, >= 10^9: never; default 500 while the cop-turn
+                                    0: always pull, >= 10^9: never; default 150 while the cop-turn
                                     bitsets fit in 256 MiB, else 190; a negative value restores it */
     CR_TUNE_PULL_TILES = 7,      /* 1 (default): the pull's stages load their input rows through LDS
                                     tiles; 0: one row per lane with direct gathers */
@@ -250,7 +250,7 @@ enum cr_tuning {
                                     stage runs k_pull_tile */
     CR_TUNE_PULL_GATE = 15,      /* levels enqueued between host synchronisations may pull only if the
                                     first one's robber frontier is above 1/gate of the pull threshold
-                                    (else they push and launch no pull kernels); 0 = always (default 16) */
+                                    (else they push and launch no pull kernels); 0 = always (default 4) */
     CR_TUNE_ROBBER_OPEN = 16,    /* the robber step finds a row's new robber-turn states from its open
                                     cop-turn states where open <= value * new cop-turn states, else from
                                     the new ones; 0..1000 (default 2) */
