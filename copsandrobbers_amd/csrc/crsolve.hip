@@ -11,7 +11,10 @@
 //   * bit planes, one row of W = ceil(N/64) 64-bit words per configuration,
 //     bit r = robber vertex r:
 //       won0 / won1  states already won with the cops / the robber to move
-//       fr1          robber-turn states won at the current depth (frontier)
+//       fr0          cop-turn states won at the latest depth
+//       fr1[2]       robber-turn states won at the current depth (frontier),
+//                    by depth parity: read by the cop step, written by the
+//                    robber step of the level before
 //       nx0          cop-turn candidates of the next depth (atomicOr target)
 //   * table: the reference's AoS DataItem bytes, index (cId*N + r)*2 + turn,
 //     written once per state when it is won.
@@ -20,7 +23,8 @@
 //
 // One depth level d -> d+1 (level-synchronous retrograde analysis; equal to
 // the reference's pass sweep, whose pass i marks cop states of depth 2i-2 and
-// 2i-1 and robber states of depth 2i-1 and 2i):
+// 2i-1 and robber states of depth 2i-1 and 2i), robber step first (both
+// halves read only depths <= d):
 //   k_cop_push      every configuration C' with new robber-turn states at
 //                   depth d ORs its robber mask into nx0 of each predecessor C
 //                   (each cop steps back along an in-edge; the product is
@@ -110,7 +114,11 @@ struct Compact {
 struct LevelArgs {
     uint64_t* won0;
     uint64_t* won1;
-    uint64_t* fr1;
+    // robber-turn frontier by depth parity: the cop step of depth d reads (and
+    // clears) fr1s[d & 1] and the robber step writes depth d + 1 into
+    // fr1s[(d + 1) & 1]. Two buffers let the robber step run first (fused
+    // finalize); otherwise both are one buffer and the cop step runs first.
+    uint64_t* fr1s[2];
     uint64_t* nx0;
     uint8_t* table;
     uint32_t* list0[2];  // per parity: configurations with new cop-turn states (sorted)
@@ -254,7 +262,8 @@ __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
                 for (int q = 0; q < W; ++q) {
                     a.won0[o + q] = cm[q] | c1[q];
                     a.won1[o + q] = cm[q] | r1[q];
-                    a.fr1[o + q] = r1[q];
+                    a.fr1s[1][o + q] = r1[q];  // depth 1, read by the cop step of level 1
+                    if (a.fr1s[0] != a.fr1s[1]) a.fr1s[0][o + q] = 0;
                     a.fr0[o + q] = c1[q];
                     a.nx0[o + q] = 0;
                     lc += __popcll(cm[q]);
@@ -367,11 +376,12 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
     const uint8_t* in_nbr = lds + gv.off_in_nbr;
     const int lane = threadIdx.x & 63;
     const int W = gv.W;
+    uint64_t* fr1 = a.fr1s[cur];
     const uint32_t* list = a.list1[cur];
     for (uint32_t it = sp.first; it < sp.end; it += sp.stride) {
         const uint32_t row = list[it];
         uint64_t m[4] = {0, 0, 0, 0};
-        for (int w = 0; w < W; ++w) m[w] = a.fr1[static_cast<uint64_t>(row) * W + w];
+        for (int w = 0; w < W; ++w) m[w] = fr1[static_cast<uint64_t>(row) * W + w];
         uint32_t cops[K], deg[K], base[K];
         unrank<K>(row, gv, binom, cops);
         uint32_t P = 1;
@@ -409,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
                                         static_cast<unsigned long long>(mw[u]));
             }
         }
-        if (lane < W) a.fr1[static_cast<uint64_t>(row) * W + lane] = 0;
+        if (lane < W) fr1[static_cast<uint64_t>(row) * W + lane] = 0;
     }
 }
 
@@ -429,6 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
 // ---------------------------------------------------------------------------
 struct SepStage {
     const uint64_t* in;   // H_{i-1}, rows [cnt(i-1) * cnt(K-i+1)]
+    const uint64_t* in_odd;  // stage 1: the frontier of odd depths (in holds the even ones)
     uint64_t* out;        // H_i, rows [cnt(i) * cnt(K-i)] (stage K: nx0)
     uint32_t cnt_b;       // cnt(K-i): B-multisets of H_i
     uint32_t cnt_b_in;    // cnt(K-i+1): B-multisets of H_{i-1}
@@ -441,6 +452,11 @@ struct SepStage {
     int32_t fuse;         // stage K: do k_cop_finalize's work instead of writing nx0
     int32_t clear;        // stage 2: clear the frontier rows of fr1 (k_clear_frontier's work)
 };
+
+// The stage's input rows: stage 1 reads the robber frontier of the level's depth.
+__device__ __forceinline__ const uint64_t* stage_input(const SepStage& st, const Ctl* ctl) {
+    return st.in_odd && (ctl->level & 1) ? st.in_odd : st.in;
+}
 
 // Loads / stores of one W-word row: 16-byte vector accesses where the row is
 // 16-byte aligned (W = 2, 4).
@@ -508,6 +524,7 @@ template <int I, int J, int W>
 __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a, SepStage st) {
     extern __shared__ __align__(16) uint8_t lds[];
     if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
+    const uint64_t* sin = stage_input(st, a.ctl);
     load_image(lds, gv.image, gv.image_bytes);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const uint16_t* out_off = reinterpret_cast<const uint16_t*>(lds + gv.off_out_off);
@@ -601,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_sep_stage(GraphView gv, LevelArgs a,
 #pragma unroll
             for (int u = 0; u < kPullMlp; ++u) {
                 if (u < m) {
-                    load_row<W>(st.in + (row_in + idx[u] * in_stride) * W, got[u]);
+                    load_row<W>(sin + (row_in + idx[u] * in_stride) * W, got[u]);
                 } else {
 #pragma unroll
                     for (int q = 0; q < W; ++q) got[u][q] = 0;
@@ -694,6 +711,7 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArg
     __shared__ uint32_t t_lo[2][kTileMaxRows], t_pre[2][kTileMaxRows + 1];
     __shared__ unsigned long long t_out[2];
     if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
+    const uint64_t* sin = stage_input(st, a.ctl);
     load_image(lds, gv.image, gv.image_bytes);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const uint16_t* out_off = reinterpret_cast<const uint16_t*>(lds + gv.off_out_off);
@@ -725,7 +743,7 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArg
         const uint32_t* list = a.list1[cur];
         for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < cw;
              i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-            a.fr1[static_cast<uint64_t>(list[i / W]) * W + i % W] = 0;
+            a.fr1s[cur][static_cast<uint64_t>(list[i / W]) * W + i % W] = 0;
     }
     if (ch >= nchunks) return;  // uniform per workgroup
 
@@ -837,7 +855,7 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArg
                         }
                         idx = rank0 - sum - binom[(J1 - q) * Mp + (M1 - 1 - v - q)];
                     }
-                    stage_w[m] = st.in[(static_cast<uint64_t>(idx) * cnt_in + ra0) * W + w];
+                    stage_w[m] = sin[(static_cast<uint64_t>(idx) * cnt_in + ra0) * W + w];
                 }
             }
         }
@@ -898,11 +916,12 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArg
                     }
                 }
                 if (st.fuse) {
-                    // k_cop_finalize's work for a pulled level: only rows with
-                    // new states are written (nx0 is zero elsewhere), and the
-                    // row flags and group counts (zeroed before the stage) take
-                    // this wave's rows. won0 and fr0 are updated after the
-                    // level's robber step, by k_log_lists.
+                    // k_cop_finalize's work for a pulled level (the level's
+                    // robber step has already read fr0): rows with new states
+                    // write them to fr0 (st.out; fr0 is read only for the rows
+                    // listed), and the row flags and group counts (zeroed
+                    // before the stage) take this wave's rows. k_log_lists
+                    // adds them to won0.
                     if (pop) store_row<W>(st.out + o * W, acc);
                     fuse_flags(a, o0 + j0 + (threadIdx.x & ~63u), __ballot(pop != 0), pop);
                 } else if (valid) {
@@ -939,6 +958,7 @@ __global__ __launch_bounds__(kTile1Block) void k_pull_tile1(GraphView gv, LevelA
     static_assert(J > 0, "stage 1 of a single-cop pull is its last stage");
     extern __shared__ __align__(16) uint8_t lds[];
     if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
+    const uint64_t* sin = stage_input(st, a.ctl);
     load_image(lds, gv.image, gv.image_bytes);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const uint16_t* out_off = reinterpret_cast<const uint16_t*>(lds + gv.off_out_off);
@@ -1011,7 +1031,7 @@ __global__ __launch_bounds__(kTile1Block) void k_pull_tile1(GraphView gv, LevelA
                     q += lt ? 1u : 0u;
                 }
                 const uint32_t idx = rank0 - sum - binom[(J1 - q) * Mp + (M1 - 1 - v - q)];
-                stage_w[m] = st.in[static_cast<uint64_t>(idx) * W + my_w];
+                stage_w[m] = sin[static_cast<uint64_t>(idx) * W + my_w];
             }
         }
     };
@@ -1061,7 +1081,7 @@ __global__ __launch_bounds__(kBlock) void k_clear_frontier(LevelArgs a, int32_t 
     const uint32_t* list = a.list1[cur];
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-        a.fr1[static_cast<uint64_t>(list[i / W]) * W + i % W] = 0;
+        a.fr1s[cur][static_cast<uint64_t>(list[i / W]) * W + i % W] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1179,7 +1199,7 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
 #pragma unroll
                 for (int q = 0; q < W; ++q) u[q] = w1[q] | nw[q];
                 store_row<W>(a.won1 + o, u);
-                store_row<W>(a.fr1 + o, nw);
+                store_row<W>(a.fr1s[cur ^ 1] + o, nw);  // depth level + 1
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
                     if (a.log_rows) continue;  // the depth bytes go through the log (k_log_lists)
@@ -1510,11 +1530,12 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
 }
 
 // The depth bytes of the states listed by k_write_lists (not in init mode):
-// one thread per list entry takes the row's new robber bits (fr1 for the
-// robber turn, fr0 for the cop turn) into the depth log, or, when the log has
-// no room for this depth, writes the bytes into the table directly. After a
-// pulled level with a fused last stage the cop-turn bits are still in nx0:
-// they are settled into won0 and fr0 here.
+// one thread per list entry takes the row's new robber bits (fr1 of the new
+// depth's parity for the robber turn, fr0 for the cop turn) into the depth
+// log, or, when the log has no room for this depth, writes the bytes into the
+// table directly. After a pulled level with a fused last stage the new
+// cop-turn states are added to won0 here (that stage wrote fr0 only; the next
+// level's robber step reads won0).
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
     const Ctl* ctl = a.ctl;
@@ -1522,7 +1543,7 @@ __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
     const uint32_t depth = ctl->level, nxt = depth & 1;
     const uint32_t n0 = ctl->cnt0[nxt], n1 = ctl->cnt1[nxt];
     const bool logging = ctl->log_now != 0;
-    const bool fused = a.fuse && ctl->dir[depth - 1] != 0;  // the level that listed these states pulled
+    const bool fused = a.fuse && ctl->dir[depth - 1] != 0;  // the level that found these states pulled
     const uint8_t nb = won_byte(depth);
     const unsigned long long off0 = ctl->log_off[depth][0], off1 = ctl->log_off[depth][1];
     const uint64_t total = static_cast<uint64_t>(n0) + n1;
@@ -1533,22 +1554,13 @@ __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
         const uint32_t row = (robber ? a.list1[nxt] : a.list0[nxt])[k];
         const uint64_t o = static_cast<uint64_t>(row) * W;
         uint64_t w[W];
-        if (!robber && fused) {
-            // a pulled level whose last stage left the new cop-turn states in
-            // nx0: settle them now that the level's robber step has run
-            uint64_t w0[W], z[W];
-            load_row<W>(a.nx0 + o, w);
+        load_row<W>((robber ? a.fr1s[nxt] : a.fr0) + o, w);
+        if (!robber && fused) {  // the last pull stage left the new cop-turn states in fr0 only
+            uint64_t w0[W];
             load_row<W>(a.won0 + o, w0);
 #pragma unroll
-            for (int q = 0; q < W; ++q) {
-                w0[q] |= w[q];
-                z[q] = 0;
-            }
+            for (int q = 0; q < W; ++q) w0[q] |= w[q];
             store_row<W>(a.won0 + o, w0);
-            store_row<W>(a.fr0 + o, w);
-            store_row<W>(a.nx0 + o, z);
-        } else {
-            load_row<W>((robber ? a.fr1 : a.fr0) + o, w);
         }
         if (logging) {
             const unsigned long long e = (robber ? off1 : off0) + k;
@@ -1745,7 +1757,7 @@ __global__ __launch_bounds__(kBlock) void k_cop_push_csr(GraphView gv, LevelArgs
     for (uint32_t it = blockIdx.x * kWavesPerBlock + wave_id(); it < count; it += gridDim.x * kWavesPerBlock) {
         const uint32_t row = list[it];
         uint64_t m[4] = {0, 0, 0, 0};
-        for (int w = 0; w < W; ++w) m[w] = a.fr1[static_cast<uint64_t>(row) * W + w];
+        for (int w = 0; w < W; ++w) m[w] = a.fr1s[cur][static_cast<uint64_t>(row) * W + w];
         const unsigned long long beg = heads[row], end = heads[row + 1];
         for (unsigned long long i = beg + lane; i < end; i += 256) {
             uint64_t pr[4];
@@ -1771,7 +1783,7 @@ __global__ __launch_bounds__(kBlock) void k_cop_push_csr(GraphView gv, LevelArgs
                                         static_cast<unsigned long long>(mw[u]));
             }
         }
-        if (lane < W) a.fr1[static_cast<uint64_t>(row) * W + lane] = 0;
+        if (lane < W) a.fr1s[cur][static_cast<uint64_t>(row) * W + lane] = 0;
     }
 }
 
@@ -1980,7 +1992,7 @@ struct cr_solver {
     uint8_t* d_table = nullptr;
     uint64_t* d_won0 = nullptr;
     uint64_t* d_won1 = nullptr;
-    uint64_t* d_fr1 = nullptr;
+    uint64_t* d_fr1[2] = {nullptr, nullptr};  // robber-turn frontier by depth parity
     uint64_t* d_fr0 = nullptr;    // per configuration: the cop-turn states its last depth added
     uint64_t* d_nx0 = nullptr;
     uint32_t* d_list[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [turn][parity] dense row lists
@@ -2067,7 +2079,8 @@ struct cr_solver {
         hipFree(d_table);
         hipFree(d_won0);
         hipFree(d_won1);
-        hipFree(d_fr1);
+        if (d_fr1[1] != d_fr1[0]) hipFree(d_fr1[1]);
+        hipFree(d_fr1[0]);
         hipFree(d_nx0);
         for (auto& t : d_list)
             for (auto& b : t) { hipFree(b); b = nullptr; }
@@ -2112,7 +2125,8 @@ struct cr_solver {
         d_inmask = nullptr;
         d_fr0 = nullptr;
         d_table = nullptr;
-        d_won0 = d_won1 = d_fr1 = d_nx0 = nullptr;
+        d_won0 = d_won1 = d_nx0 = nullptr;
+        d_fr1[0] = d_fr1[1] = nullptr;
         d_ctl = nullptr;
         d_image = nullptr;
         d_outmask = nullptr;
@@ -2309,7 +2323,8 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
         const void* fn = reinterpret_cast<const void*>(kern);
         st.stage = I;
         st.in = in;
-        st.out = I == K ? s->d_nx0 : s->d_sep[(I - 1) & 1];
+        st.in_odd = I == 1 ? s->d_fr1[1] : nullptr;
+        st.out = I < K ? s->d_sep[(I - 1) & 1] : st.fuse ? s->d_fr0 : s->d_nx0;
         st.cnt_b = static_cast<uint32_t>(s->ms_count[K - I]);
         st.cnt_b_in = static_cast<uint32_t>(s->ms_count[K - I + 1]);
         st.cnt_a = static_cast<uint32_t>(s->ms_count[I]);
@@ -2341,7 +2356,7 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
 template <int K, int W>
 int launch_pull(cr_solver* s, LevelArgs a) {
     s->stage2_cleared = false;
-    if (int rc = launch_pull_stage<K, W, 1>(s, a, s->d_fr1)) return rc;
+    if (int rc = launch_pull_stage<K, W, 1>(s, a, s->d_fr1[0])) return rc;
     if (s->stage2_cleared) return 0;  // the tiled stage 2 cleared the frontier
     Timer* t = start_timer(s, "clear_frontier");
     hipLaunchKernelGGL(k_clear_frontier, dim3(1024), dim3(kBlock), 0, s->stream, a, s->W);
@@ -2406,14 +2421,17 @@ LevelArgs level_args(cr_solver* s) {
     LevelArgs a{};
     a.won0 = s->d_won0;
     a.won1 = s->d_won1;
-    a.fr1 = s->d_fr1;
+    a.fr1s[0] = s->d_fr1[0];
+    a.fr1s[1] = s->d_fr1[1];
     a.nx0 = s->d_nx0;
     a.table = s->d_table;
     a.log_rows = s->d_log_rows;
     a.log_bits = s->d_log_bits;
     a.n = s->g.n;
     a.W = s->W;
-    a.fuse = s->d_log_rows != nullptr && s->pull_tiles && s->fuse_finalize && s->fuse_ok;
+    // fused finalize: the depth log, LDS tiles, and two robber-frontier
+    // buffers (the robber step runs before the cop step)
+    a.fuse = s->d_log_rows != nullptr && s->pull_tiles && s->fuse_finalize && s->fuse_ok && s->d_fr1[1] != s->d_fr1[0];
     for (int b = 0; b < 2; ++b) {
         a.list0[b] = s->d_list[0][b];
         a.list1[b] = s->d_list[1][b];
@@ -2582,11 +2600,14 @@ uint32_t log_chunk_rows(const cr_solver* s);
 int alloc_log(cr_solver* s) {
     if (!s->log_factor || s->C < s->log_min_configs) return 0;
     const uint64_t entry = 4 + 8ull * s->W;
+    int rc = 0;
+    // the second robber-frontier buffer lets the robber step run before the
+    // cop step, which the fused finalize of logged solves needs
+    if (!(s->d_fr1[1] = dev_alloc<uint64_t>(s->C * s->W, s->device_bytes, rc))) return rc;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     const uint64_t cap = std::min<uint64_t>(static_cast<uint64_t>(s->log_factor) * s->C, freeb / 2 / entry);
     if (cap < 2 * s->C) return 0;
-    int rc = 0;
     if (!(s->d_log_rows = dev_alloc<uint32_t>(cap, s->device_bytes, rc))) return rc;
     if (!(s->d_log_bits = dev_alloc<uint64_t>(cap * s->W, s->device_bytes, rc))) return rc;
     if (!(s->d_lists = dev_alloc<LogList>(kMaxLogLists, s->device_bytes, rc))) return rc;
@@ -2775,7 +2796,8 @@ int do_build(cr_solver* s) {
     if (!(s->d_table = dev_alloc<uint8_t>(s->S, s->device_bytes, rc))) return rc;
     if (!(s->d_won0 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
     if (!(s->d_won1 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
-    if (!(s->d_fr1 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
+    if (!(s->d_fr1[0] = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
+    s->d_fr1[1] = s->d_fr1[0];  // a second buffer only with the depth log (alloc_log)
     if (!(s->d_fr0 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
     if (!(s->d_nx0 = dev_alloc<uint64_t>(plane, s->device_bytes, rc))) return rc;
     for (auto& t : s->d_list)
@@ -2993,15 +3015,31 @@ int do_loop(cr_solver* s) {
         }
         if (!pulls) la.pull_rows = 0xffffffffu;
         for (int l = 0; l < s->levels_per_sync; ++l) {
-            Timer* t = start_timer(s, "cop_push", true);
+            // The robber step (depth d + 1 robber-turn states from cop-turn
+            // states of depth <= d) and the cop step (depth d + 1 cop-turn
+            // states from the depth-d robber frontier) read only depths <= d.
+            // Where the last pull stage settles won0 and fr0 itself (fused
+            // finalize), the robber step goes first; otherwise the cop step
+            // does, right after the lists that made its frontier (still in
+            // the caches on small tables), and k_cop_finalize settles after
+            // both.
+            Timer* t = nullptr;
+            if (la.fuse) {
+                t = start_timer(s, "robber_step");
+                if ((rc = dispatch_w<LaunchRobber>(s->W, s, la)) != 0) return rc;
+                t->stop();
+            }
+            t = start_timer(s, "cop_push", true);
             s->launch_timer = t;
             rc = dispatch_k<LaunchPush>(s->k, s, la);
             s->launch_timer = nullptr;
             if (rc) return rc;
             if (pulls && (rc = dispatch_k<LaunchPull>(s->k, s, la)) != 0) return rc;
-            t = start_timer(s, "robber_step");
-            if ((rc = dispatch_w<LaunchRobber>(s->W, s, la)) != 0) return rc;
-            t->stop();
+            if (!la.fuse) {
+                t = start_timer(s, "robber_step");
+                if ((rc = dispatch_w<LaunchRobber>(s->W, s, la)) != 0) return rc;
+                t->stop();
+            }
             t = start_timer(s, "cop_finalize");
             if ((rc = dispatch_w<LaunchFinalize>(s->W, s, la)) != 0) return rc;
             t->stop();
