@@ -920,8 +920,8 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArg
                     // robber step has already read fr0): rows with new states
                     // write them to fr0 (st.out; fr0 is read only for the rows
                     // listed), and the row flags and group counts (zeroed
-                    // before the stage) take this wave's rows. k_log_lists
-                    // adds them to won0.
+                    // before the stage) take this wave's rows. The next
+                    // level's robber step adds them to won0.
                     if (pop) store_row<W>(st.out + o * W, acc);
                     fuse_flags(a, o0 + j0 + (threadIdx.x & ~63u), __ballot(pop != 0), pop);
                 } else if (valid) {
@@ -1126,6 +1126,19 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
                 load_row<W>(a.won0 + o, w0);
                 load_row<W>(a.won1 + o, w1);
                 load_row<W>(a.fr0 + o, f0);
+            }
+            // After a pulled level with a fused last stage the new cop-turn
+            // states are in fr0 only: take them into won0 here (a no-op after
+            // other levels), before anything of this level reads won0.
+            if (live) {
+                bool add0 = false;
+#pragma unroll
+                for (int q = 0; q < W; ++q) add0 |= (f0[q] & ~w0[q]) != 0;
+                if (add0) {
+#pragma unroll
+                    for (int q = 0; q < W; ++q) w0[q] |= f0[q];
+                    store_row<W>(a.won0 + o, w0);
+                }
             }
             uint32_t nopen = 0, nnew = 0;
 #pragma unroll
@@ -1533,9 +1546,7 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
 // one thread per list entry takes the row's new robber bits (fr1 of the new
 // depth's parity for the robber turn, fr0 for the cop turn) into the depth
 // log, or, when the log has no room for this depth, writes the bytes into the
-// table directly. After a pulled level with a fused last stage the new
-// cop-turn states are added to won0 here (that stage wrote fr0 only; the next
-// level's robber step reads won0).
+// table directly.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
     const Ctl* ctl = a.ctl;
@@ -1543,7 +1554,6 @@ __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
     const uint32_t depth = ctl->level, nxt = depth & 1;
     const uint32_t n0 = ctl->cnt0[nxt], n1 = ctl->cnt1[nxt];
     const bool logging = ctl->log_now != 0;
-    const bool fused = a.fuse && ctl->dir[depth - 1] != 0;  // the level that found these states pulled
     const uint8_t nb = won_byte(depth);
     const unsigned long long off0 = ctl->log_off[depth][0], off1 = ctl->log_off[depth][1];
     const uint64_t total = static_cast<uint64_t>(n0) + n1;
@@ -1555,13 +1565,6 @@ __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
         const uint64_t o = static_cast<uint64_t>(row) * W;
         uint64_t w[W];
         load_row<W>((robber ? a.fr1s[nxt] : a.fr0) + o, w);
-        if (!robber && fused) {  // the last pull stage left the new cop-turn states in fr0 only
-            uint64_t w0[W];
-            load_row<W>(a.won0 + o, w0);
-#pragma unroll
-            for (int q = 0; q < W; ++q) w0[q] |= w[q];
-            store_row<W>(a.won0 + o, w0);
-        }
         if (logging) {
             const unsigned long long e = (robber ? off1 : off0) + k;
             a.log_rows[e] = row;
