@@ -737,7 +737,7 @@ __global__ __launch_bounds__(kTileBlock) void k_pull_tile(GraphView gv, LevelArg
     // repeats find it in the caches.
     const uint64_t nchunks = (ntiles + st.chunk - 1) / st.chunk;
     uint64_t ch = blockIdx.x;
-    if (st.clear) {  // stage 2: stage 1 has read the frontier; clear its rows for the robber step
+    if (st.clear) {  // stage 2: stage 1 has read the frontier; clear its rows (k_clear_frontier's work)
         const uint32_t cur = a.ctl->level & 1;
         const uint64_t cw = static_cast<uint64_t>(a.ctl->cnt1[cur]) * W;
         const uint32_t* list = a.list1[cur];
@@ -1073,7 +1073,8 @@ __global__ __launch_bounds__(kTile1Block) void k_pull_tile1(GraphView gv, LevelA
 }
 
 // After a pull, the frontier rows still hold their depth-d robber bits;
-// clear them before the robber step writes depth d+1 into its rows.
+// clear them: the robber step writes depth d+2 into this buffer one level
+// later (or depth d+1 right after, where both parities share one buffer).
 __global__ __launch_bounds__(kBlock) void k_clear_frontier(LevelArgs a, int32_t W) {
     if (a.ctl->done || !a.ctl->pull) return;
     const uint32_t cur = a.ctl->level & 1;
@@ -3021,11 +3022,10 @@ int do_loop(cr_solver* s) {
             // The robber step (depth d + 1 robber-turn states from cop-turn
             // states of depth <= d) and the cop step (depth d + 1 cop-turn
             // states from the depth-d robber frontier) read only depths <= d.
-            // Where the last pull stage settles won0 and fr0 itself (fused
-            // finalize), the robber step goes first; otherwise the cop step
-            // does, right after the lists that made its frontier (still in
-            // the caches on small tables), and k_cop_finalize settles after
-            // both.
+            // Where the last pull stage writes fr0 itself (fused finalize),
+            // the robber step goes first; otherwise the cop step does, right
+            // after the lists that made its frontier (still in the caches on
+            // small tables), and k_cop_finalize settles after both.
             Timer* t = nullptr;
             if (la.fuse) {
                 t = start_timer(s, "robber_step");
