@@ -1,0 +1,56 @@
+"""Randomised parity of the plain solve (v1, p = 1) with the oracle: random small
+graphs (6..40 vertices, 1..4 cops) under random speed settings, none of which
+may change a table. Fixed seeds; every case prints its graph size and settings
+before it runs."""
+import numpy as np
+import pytest
+
+import copsandrobbers_amd as cra
+
+pytestmark = pytest.mark.gpu
+
+CHOICES = {
+    "pull_scale": [-1, 0, 10**9, 150],
+    "depth_log": [40, 2, 0],
+    "depth_log_min": [0, 1 << 24],
+    "fuse_finalize": [0, 1],
+    "pull_tiles": [0, 1, 1],
+    "pull_tile1": [0, 3, 9],
+    "pull_tile_rows": [2, 4],
+    "pull_tile_chunk": [1, 16],
+    "pull_gate": [0, 1, 4, 16],
+    "robber_open": [0, 2, 1000],
+    "levels_per_sync": [1, 2, 3, 5, 8],
+}
+
+
+def random_graph(rng):
+    """A random undirected graph in which every vertex has a move."""
+    while True:
+        n = int(rng.integers(6, 41))
+        adj = (rng.random((n, n)) < rng.uniform(0.06, 0.35)).astype(np.uint8)
+        adj = np.triu(adj, 1)
+        adj = adj | adj.T
+        if adj.sum(1).min() > 0:
+            return n, adj
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_random_graphs_and_settings_match_oracle(oracle, seed):
+    rng = np.random.default_rng(seed)
+    s = cra.Solver(0)
+    try:
+        for case in range(30):
+            n, adj = random_graph(rng)
+            k = int(rng.integers(1, 5 if n <= 24 else 4))
+            tuning = {name: int(rng.choice(values)) for name, values in CHOICES.items()}
+            print(f"random case {seed}/{case}: n={n} k={k} {tuning}", flush=True)
+            s.set_tuning(**tuning)
+            s.load_graph_matrix(adj, k, 1)
+            s.build_aux_graph()
+            s.initialize_captures()
+            s.main_loop()
+            want = oracle.v1_solve(adj, k, 1).states
+            assert np.array_equal(s.states(), want), (seed, case, n, k, tuning)
+    finally:
+        s.close()
