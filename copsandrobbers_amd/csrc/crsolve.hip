@@ -138,6 +138,12 @@ struct LevelArgs {
     int32_t n, W;
     int32_t fuse;              // pulled levels: the last pull stage does k_cop_finalize's work
     uint32_t open_factor;      // robber step: rows with open <= open_factor * new cop-turn states take the open-state path
+    // two-level scan of the group counts (0: one workgroup scans them all):
+    // k_scan_tiles scans tiles of scan_tile groups, k_advance the tile totals
+    uint32_t scan_tile;
+    uint32_t tile_stride;                 // entries per list in tile_cnt / tile_states
+    uint32_t* tile_cnt;                   // [2][tile_stride]: tile totals, then offsets
+    unsigned long long* tile_states;      // [2][tile_stride]
 };
 
 // Work items [0, n) in 8 contiguous segments, one per XCD (workgroups are
@@ -1309,6 +1315,8 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
 // consecutive counts (independent loads), the tile is scanned with wave
 // shuffles and one pass over the wave totals.
 constexpr int kScanPer = 8;
+constexpr uint32_t kScanTileMax = kScanThreads * kScanPer;  // groups one pass of scan_groups_n takes
+constexpr uint32_t kScanTileMin = 4;
 
 // L lists of group counts are scanned in the same passes, so their loads are
 // in flight together (k_advance scans the cop-turn and robber-turn lists).
@@ -1398,6 +1406,27 @@ __device__ void scan_groups(uint32_t* gcnt, const unsigned long long* gstates, u
 // adds the level's state counts to the histogram and moves to depth level+1;
 // stops when no row holds a new state or when the next depth would have to
 // reach the 127 cap (levels run d = 1..125).
+// First half of the two-level scan (large tables): workgroup b scans tile
+// b mod T of list b / T (T tiles per list), leaving offsets within the tile in
+// gcnt and the tile's totals in tile_cnt / tile_states for k_advance.
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(LevelArgs a, uint32_t T) {
+    const Ctl* ctl = a.ctl;
+    if (ctl->done) return;  // uniform
+    const uint32_t l = blockIdx.x / T, t = blockIdx.x - l * T;
+    const uint32_t ng = l == 0 ? (a.nbatch + kGroupWords - 1) / kGroupWords
+                               : ((ctl->cnt0[ctl->level & 1] + 63) / 64 + kGroupWords - 1) / kGroupWords;
+    const uint32_t base = t * a.scan_tile;
+    if (base >= ng) return;  // uniform
+    const Compact& c = l == 0 ? a.c0 : a.c1;
+    uint32_t total = 0;
+    unsigned long long states = 0;
+    scan_groups(c.gcnt + base, c.gstates + base, min(a.scan_tile, ng - base), total, states);
+    if (threadIdx.x == 0) {
+        a.tile_cnt[l * a.tile_stride + t] = total;
+        a.tile_states[l * a.tile_stride + t] = states;
+    }
+}
+
 __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
     Ctl* ctl = a.ctl;
     if (ctl->done) {  // a queued level after the last one: its k_write_lists does nothing
@@ -1408,12 +1437,19 @@ __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
     const uint32_t ng0 = (a.nbatch + kGroupWords - 1) / kGroupWords;
     const uint32_t nw1 = (ctl->cnt0[cur] + 63) / 64;
     const uint32_t ng1 = (nw1 + kGroupWords - 1) / kGroupWords;
-    uint32_t* gc[2] = {a.c0.gcnt, a.c1.gcnt};
-    const unsigned long long* gs[2] = {a.c0.gstates, a.c1.gstates};
-    const uint32_t ng[2] = {ng0, ng1};
     uint32_t tt[2] = {0, 0};
     unsigned long long ss[2] = {0, 0};
-    scan_groups_n<2>(gc, gs, ng, tt, ss);
+    if (a.scan_tile) {  // k_scan_tiles has scanned within tiles: scan the tile totals
+        uint32_t* gc[2] = {a.tile_cnt, a.tile_cnt + a.tile_stride};
+        const unsigned long long* gs[2] = {a.tile_states, a.tile_states + a.tile_stride};
+        const uint32_t nt[2] = {(ng0 + a.scan_tile - 1) / a.scan_tile, (ng1 + a.scan_tile - 1) / a.scan_tile};
+        scan_groups_n<2>(gc, gs, nt, tt, ss);
+    } else {
+        uint32_t* gc[2] = {a.c0.gcnt, a.c1.gcnt};
+        const unsigned long long* gs[2] = {a.c0.gstates, a.c1.gstates};
+        const uint32_t ng[2] = {ng0, ng1};
+        scan_groups_n<2>(gc, gs, ng, tt, ss);
+    }
     const uint32_t t0 = tt[0], t1 = tt[1];
     const unsigned long long s0 = ss[0], s1 = ss[1];
     if (threadIdx.x == 0) {
@@ -1506,7 +1542,12 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
             const uint32_t v = __shfl_up(incl, off, kGroupWords);
             if (static_cast<int>(word % kGroupWords) >= off) incl += v;
         }
-        const uint32_t pos = bits ? c.gcnt[word / kGroupWords] + incl - pc : 0u;
+        uint32_t pos = 0;
+        if (bits) {
+            const uint32_t g = word / kGroupWords;
+            pos = c.gcnt[g] + incl - pc;
+            if (a.scan_tile && !init_mode) pos += a.tile_cnt[(space1 ? a.tile_stride : 0u) + g / a.scan_tile];
+        }
         uint32_t* dst = space1 ? a.list1[nxt] : a.list0[nxt];
         unsigned long long live = __ballot(bits != 0);
         if (!live) continue;  // uniform
@@ -2037,6 +2078,10 @@ struct cr_solver {
     uint32_t tile_chunk = 16;                 // k_pull_tile: consecutive tiles a workgroup takes at a time
     uint32_t tile1_rows = 9;                  // k_pull_tile1 (stage 1 of a tiled multi-cop pull): B per tile; 0 = off
     uint32_t open_factor = 2;                 // robber step: open-state path where open <= factor * new
+    uint32_t scan_tile = kScanTileMax;        // groups per tile of the two-level level scan
+    uint32_t tile_stride = 0;
+    uint32_t* d_tile_cnt = nullptr;
+    unsigned long long* d_tile_states = nullptr;
     uint32_t pull_gate = 4;                   // a chunk may pull if its first frontier * pull_gate > pull_rows
     bool fuse_finalize = true;                // tuning: with the depth log, pulled levels finalize in the last stage
     bool fuse_ok = false;                     // the last pull stage runs as LDS tiles (image and one row per vertex fit)
@@ -2097,6 +2142,10 @@ struct cr_solver {
             d_gstates[i] = nullptr;
         }
         hipFree(d_gcaps);
+        hipFree(d_tile_cnt);
+        hipFree(d_tile_states);
+        d_tile_cnt = nullptr;
+        d_tile_states = nullptr;
         d_gcaps = nullptr;
         hipFree(d_heads);
         hipFree(d_csr);
@@ -2451,6 +2500,11 @@ LevelArgs level_args(cr_solver* s) {
     a.fr0 = s->d_fr0;
     a.filter = s->filter;
     a.open_factor = s->open_factor;
+    // two-level level scan where the groups span more than one tile
+    a.scan_tile = s->ngroups() > s->scan_tile ? s->scan_tile : 0u;
+    a.tile_stride = s->tile_stride;
+    a.tile_cnt = s->d_tile_cnt;
+    a.tile_states = s->d_tile_states;
     return a;
 }
 
@@ -2814,6 +2868,9 @@ int do_build(cr_solver* s) {
         if (!(s->d_gstates[i] = dev_alloc<unsigned long long>(ngroups, s->device_bytes, rc))) return rc;
     }
     if (!(s->d_gcaps = dev_alloc<unsigned long long>(ngroups, s->device_bytes, rc))) return rc;
+    s->tile_stride = static_cast<uint32_t>(ngroups / kScanTileMin + 1);  // tiles at the smallest tile size
+    if (!(s->d_tile_cnt = dev_alloc<uint32_t>(2ull * s->tile_stride, s->device_bytes, rc))) return rc;
+    if (!(s->d_tile_states = dev_alloc<unsigned long long>(2ull * s->tile_stride, s->device_bytes, rc))) return rc;
     if (!(s->d_ctl = dev_alloc<Ctl>(1, s->device_bytes, rc))) return rc;
     if (!s->h_ctl) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_ctl), sizeof(Ctl), hipHostMallocDefault));
     if ((rc = upload_graph(s)) != 0) return rc;
@@ -3046,6 +3103,10 @@ int do_loop(cr_solver* s) {
             t = start_timer(s, "cop_finalize");
             if ((rc = dispatch_w<LaunchFinalize>(s->W, s, la)) != 0) return rc;
             t->stop();
+            if (la.scan_tile) {
+                const uint32_t T = static_cast<uint32_t>((s->ngroups() + la.scan_tile - 1) / la.scan_tile);
+                hipLaunchKernelGGL(k_scan_tiles, dim3(2 * T), dim3(kScanThreads), 0, s->stream, la, T);
+            }
             hipLaunchKernelGGL(k_advance, dim3(1), dim3(kScanThreads), 0, s->stream, la);
             hipLaunchKernelGGL(k_write_lists, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, la, 0);
             if (s->log_cap && (rc = dispatch_w<LaunchLogLists>(s->W, s, la)) != 0) return rc;
@@ -3587,6 +3648,10 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             s->tile_blocks = static_cast<uint32_t>(value);
             return 0;
         case CR_TUNE_FUSE_FINALIZE: s->fuse_finalize = value != 0; return 0;
+        case CR_TUNE_SCAN_TILE:
+            if (value < kScanTileMin || value > kScanTileMax) return fail(CR_ERR_ARG, "scan tile %lld", static_cast<long long>(value));
+            s->scan_tile = static_cast<uint32_t>(value);
+            return 0;
         case CR_TUNE_PULL_TILE1:
             if (value < 0 || value > kTileMaxRows) return fail(CR_ERR_ARG, "stage-1 tile rows %lld", static_cast<long long>(value));
             s->tile1_rows = static_cast<uint32_t>(value);

@@ -254,8 +254,10 @@ enum cr_tuning {
     CR_TUNE_ROBBER_OPEN = 16,    /* the robber step finds a row's new robber-turn states from its open
                                     cop-turn states where open <= value * new cop-turn states, else from
                                     the new ones; 0..1000 (default 2) */
-    CR_TUNE_LOG_BLOCKS = 17      /* workgroups of k_log_lists, which appends the level lists to the
+    CR_TUNE_LOG_BLOCKS = 17,     /* workgroups of k_log_lists, which appends the level lists to the
                                     depth log (default 4096) */
+    CR_TUNE_SCAN_TILE = 18       /* list groups per tile of the level scan, 4..8192 (default 8192): with
+                                    more groups than one tile, tiles are scanned in parallel first */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:

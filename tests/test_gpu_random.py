@@ -21,6 +21,7 @@ CHOICES = {
     "pull_gate": [0, 1, 4, 16],
     "robber_open": [0, 2, 1000],
     "levels_per_sync": [1, 2, 3, 5, 8],
+    "scan_tile": [4, 7, 8192],
 }
 
 
