@@ -138,6 +138,7 @@ struct LevelArgs {
     int32_t n, W;
     int32_t fuse;              // pulled levels: the last pull stage does k_cop_finalize's work
     uint32_t open_factor;      // robber step: rows with open <= open_factor * new cop-turn states take the open-state path
+    uint8_t* dirty;            // per group of 256 configurations: the push wrote nx0 rows there (null: not kept)
     // two-level scan of the group counts (0: one workgroup scans them all):
     // k_scan_tiles scans tiles of scan_tile groups, k_advance the tile totals
     uint32_t scan_tile;
@@ -203,6 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
     const uint32_t ngroups = (a.nbatch + kGroupWords - 1) / kGroupWords;
     for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
         unsigned long long caps = 0, cop1 = 0, rob1 = 0, rows0 = 0, rows1 = 0;
+        if (a.dirty && threadIdx.x == 0) a.dirty[g] = 0;
         for (int j = 0; j < kWordsPerWave; ++j) {
             const uint32_t b = g * kGroupWords + wid * kWordsPerWave + j;
             if (b >= a.nbatch) break;
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
         // two predecessors per lane per iteration keep two filter loads in flight
         for (uint32_t t = lane; t < P; t += 128) {
             uint32_t tup[2][K];
-            bool keep[2];
+            bool keep[2], hit[2] = {false, false};
             uint64_t pr[2];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -421,8 +423,16 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
                 }
 #pragma unroll
                 for (int u = 0; u < 2; ++u)
-                    if (mw[u]) atomicOr(reinterpret_cast<unsigned long long*>(a.nx0 + pr[u] * W + w),
-                                        static_cast<unsigned long long>(mw[u]));
+                    if (mw[u]) {
+                        atomicOr(reinterpret_cast<unsigned long long*>(a.nx0 + pr[u] * W + w),
+                                 static_cast<unsigned long long>(mw[u]));
+                        hit[u] = true;
+                    }
+            }
+            if (a.dirty) {  // k_cop_finalize reads only the groups marked here
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+                    if (hit[u]) a.dirty[pr[u] >> 8] = 1;
             }
         }
         if (lane < W) fr1[static_cast<uint64_t>(row) * W + lane] = 0;
@@ -1255,7 +1265,18 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
     const uint8_t nb = won_byte(a.ctl->level + 1);
     const uint32_t ngroups = (a.nbatch + kGroupWords - 1) / kGroupWords;
     const uint32_t wid = wave_id();
+    // after a push, groups the push did not mark have no candidates
+    const bool marked_only = a.dirty != nullptr && !a.ctl->pull;
     for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        if (marked_only && !a.dirty[g]) {  // uniform per workgroup
+            const uint32_t b = g * kGroupWords + wid;
+            if (lane == 0 && b < a.nbatch) a.c0.flags[b] = 0;
+            if (threadIdx.x == 0) {
+                a.c0.gcnt[g] = 0;
+                a.c0.gstates[g] = 0;
+            }
+            continue;
+        }
         unsigned long long rows = 0, found = 0;
         for (int j = 0; j < kWordsPerWave; ++j) {
             const uint32_t b = g * kGroupWords + wid * kWordsPerWave + j;
@@ -1305,6 +1326,7 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
         if (threadIdx.x == 0) {
             a.c0.gcnt[g] = static_cast<uint32_t>(rows);
             a.c0.gstates[g] = found;
+            if (marked_only) a.dirty[g] = 0;  // every thread has read the mark (block_sum2 synchronised)
         }
     }
 }
@@ -1806,7 +1828,7 @@ __global__ __launch_bounds__(kBlock) void k_cop_push_csr(GraphView gv, LevelArgs
         const unsigned long long beg = heads[row], end = heads[row + 1];
         for (unsigned long long i = beg + lane; i < end; i += 256) {
             uint64_t pr[4];
-            bool keep[4];
+            bool keep[4], hit[4] = {false, false, false, false};
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 keep[u] = i + 64 * u < end;
@@ -1824,8 +1846,16 @@ __global__ __launch_bounds__(kBlock) void k_cop_push_csr(GraphView gv, LevelArgs
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    if (mw[u]) atomicOr(reinterpret_cast<unsigned long long*>(a.nx0 + pr[u] * W + w),
-                                        static_cast<unsigned long long>(mw[u]));
+                    if (mw[u]) {
+                        atomicOr(reinterpret_cast<unsigned long long*>(a.nx0 + pr[u] * W + w),
+                                 static_cast<unsigned long long>(mw[u]));
+                        hit[u] = true;
+                    }
+            }
+            if (a.dirty) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (hit[u]) a.dirty[pr[u] >> 8] = 1;
             }
         }
         if (lane < W) a.fr1s[cur][static_cast<uint64_t>(row) * W + lane] = 0;
@@ -2081,6 +2111,8 @@ struct cr_solver {
     uint32_t scan_tile = kScanTileMax;        // groups per tile of the two-level level scan
     uint32_t tile_stride = 0;
     uint32_t* d_tile_cnt = nullptr;
+    uint8_t* d_dirty = nullptr;               // per group: marked by the push for k_cop_finalize
+    bool push_dirty = true;
     unsigned long long* d_tile_states = nullptr;
     uint32_t pull_gate = 4;                   // a chunk may pull if its first frontier * pull_gate > pull_rows
     bool fuse_finalize = true;                // tuning: with the depth log, pulled levels finalize in the last stage
@@ -2143,6 +2175,8 @@ struct cr_solver {
         }
         hipFree(d_gcaps);
         hipFree(d_tile_cnt);
+        hipFree(d_dirty);
+        d_dirty = nullptr;
         hipFree(d_tile_states);
         d_tile_cnt = nullptr;
         d_tile_states = nullptr;
@@ -2505,6 +2539,7 @@ LevelArgs level_args(cr_solver* s) {
     a.tile_stride = s->tile_stride;
     a.tile_cnt = s->d_tile_cnt;
     a.tile_states = s->d_tile_states;
+    a.dirty = s->push_dirty ? s->d_dirty : nullptr;
     return a;
 }
 
@@ -2870,6 +2905,7 @@ int do_build(cr_solver* s) {
     if (!(s->d_gcaps = dev_alloc<unsigned long long>(ngroups, s->device_bytes, rc))) return rc;
     s->tile_stride = static_cast<uint32_t>(ngroups / kScanTileMin + 1);  // tiles at the smallest tile size
     if (!(s->d_tile_cnt = dev_alloc<uint32_t>(2ull * s->tile_stride, s->device_bytes, rc))) return rc;
+    if (!(s->d_dirty = dev_alloc<uint8_t>(ngroups, s->device_bytes, rc))) return rc;
     if (!(s->d_tile_states = dev_alloc<unsigned long long>(2ull * s->tile_stride, s->device_bytes, rc))) return rc;
     if (!(s->d_ctl = dev_alloc<Ctl>(1, s->device_bytes, rc))) return rc;
     if (!s->h_ctl) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_ctl), sizeof(Ctl), hipHostMallocDefault));
@@ -3648,6 +3684,7 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             s->tile_blocks = static_cast<uint32_t>(value);
             return 0;
         case CR_TUNE_FUSE_FINALIZE: s->fuse_finalize = value != 0; return 0;
+        case CR_TUNE_PUSH_DIRTY: s->push_dirty = value != 0; return 0;
         case CR_TUNE_SCAN_TILE:
             if (value < kScanTileMin || value > kScanTileMax) return fail(CR_ERR_ARG, "scan tile %lld", static_cast<long long>(value));
             s->scan_tile = static_cast<uint32_t>(value);

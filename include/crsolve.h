@@ -256,8 +256,10 @@ enum cr_tuning {
                                     the new ones; 0..1000 (default 2) */
     CR_TUNE_LOG_BLOCKS = 17,     /* workgroups of k_log_lists, which appends the level lists to the
                                     depth log (default 4096) */
-    CR_TUNE_SCAN_TILE = 18       /* list groups per tile of the level scan, 4..8192 (default 8192): with
+    CR_TUNE_SCAN_TILE = 18,      /* list groups per tile of the level scan, 4..8192 (default 8192): with
                                     more groups than one tile, tiles are scanned in parallel first */
+    CR_TUNE_PUSH_DIRTY = 19      /* 1 (default): the push marks the groups of 256 configurations it
+                                    writes, and the finalize of a push level reads only those */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:

@@ -212,6 +212,7 @@ PULL_SETTINGS = [{"pull_scale": 0, "pull_tiles": 1, "pull_tile_rows": 4},
                  {"robber_open": 1000},
                  {"row_blocks": 3, "log_blocks": 5, "depth_log": 40, "depth_log_min": 0},
                  {"scan_tile": 4, "levels_per_sync": 3},
+                 {"push_dirty": 0, "pull_scale": 10**9},
                  {"pull_scale": 0, "pull_tiles": 0},
                  {"pull_scale": 10**9},
                  {"depth_log": 0},
@@ -219,10 +220,10 @@ PULL_SETTINGS = [{"pull_scale": 0, "pull_tiles": 1, "pull_tile_rows": 4},
                  {"pull_scale": 0, "depth_log": 40, "depth_log_min": 0, "fuse_finalize": 0},
                  {"depth_log": 40, "depth_log_min": 0},
                  {"pull_scale": 0, "depth_log": 2, "depth_log_min": 0, "levels_per_sync": 3}]
-PULL_IDS = ["pull_tiles4", "pull_tiles3_chunk5", "pull_tile1_off", "pull_tile1_3", "pull_gate1", "robber_by_new", "robber_by_open", "small_grids", "scan_tiles4", "pull_rows", "push_only", "direct_bytes", "pull_log_fused",
+PULL_IDS = ["pull_tiles4", "pull_tiles3_chunk5", "pull_tile1_off", "pull_tile1_3", "pull_gate1", "robber_by_new", "robber_by_open", "small_grids", "scan_tiles4", "push_no_marks", "pull_rows", "push_only", "direct_bytes", "pull_log_fused",
             "pull_log_unfused", "depth_log", "small_log"]
 DEFAULTS = {"pull_scale": -1, "pull_tiles": 1, "pull_tile_rows": 4, "pull_tile_chunk": 16, "pull_tile_blocks": 8192, "depth_log": 40, "depth_log_min": 1 << 24,
-            "levels_per_sync": 8, "fuse_finalize": 1, "pull_tile1": 9, "pull_gate": 4, "robber_open": 2, "row_blocks": 32768, "log_blocks": 4096, "scan_tile": 8192}
+            "levels_per_sync": 8, "fuse_finalize": 1, "pull_tile1": 9, "pull_gate": 4, "robber_open": 2, "row_blocks": 32768, "log_blocks": 4096, "scan_tile": 8192, "push_dirty": 1}
 PULL_CASES = ["v1__cycle100__k2__p1", "v1__dodecahedron__k3__p1", "v1__grid36__k3__p1", "v1__grid9__k3__p1",
               "v1__line3__k2__p1", "v1__outerplanar8__k3__p1", "v1__peterson__k3__p1", "v1__planar32__k3__p1",
               "v1__scotlandyard-all__k2__p1", "v1__test__k2__p1"]

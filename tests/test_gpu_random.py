@@ -22,6 +22,7 @@ CHOICES = {
     "robber_open": [0, 2, 1000],
     "levels_per_sync": [1, 2, 3, 5, 8],
     "scan_tile": [4, 7, 8192],
+    "push_dirty": [0, 1],
 }
 
 
