@@ -1754,14 +1754,19 @@ __global__ __launch_bounds__(kBlock) void k_apply_log(const LogList* lists, int3
         // R0 * 2n is a multiple of 16), then the tail bytes
         uint8_t* trow = table + static_cast<uint64_t>(R0) * rowbytes;
         const uint32_t nbytes = (R1 - R0) * rowbytes, nvec = nbytes / 16;
-        for (uint32_t i0 = threadIdx.x; i0 < nvec; i0 += blockDim.x * 4) {  // four loads in flight
-            uint4 x[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (i0 + u * blockDim.x < nvec) x[u] = reinterpret_cast<const uint4*>(trow)[i0 + u * blockDim.x];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (i0 + u * blockDim.x < nvec) reinterpret_cast<uint4*>(lds)[i0 + u * blockDim.x] = x[u];
+        {
+            const uint4* src = reinterpret_cast<const uint4*>(trow);
+            uint4* dst = reinterpret_cast<uint4*>(lds);
+            const uint32_t bd = blockDim.x;
+            uint32_t i0 = threadIdx.x;
+            for (; i0 + 3 * bd < nvec; i0 += 4 * bd) {  // four loads in flight, no conditions
+                const uint4 x0 = src[i0], x1 = src[i0 + bd], x2 = src[i0 + 2 * bd], x3 = src[i0 + 3 * bd];
+                dst[i0] = x0;
+                dst[i0 + bd] = x1;
+                dst[i0 + 2 * bd] = x2;
+                dst[i0 + 3 * bd] = x3;
+            }
+            for (; i0 < nvec; i0 += bd) dst[i0] = src[i0];
         }
         for (uint32_t i = nvec * 16 + threadIdx.x; i < nbytes; i += blockDim.x) lds[i] = trow[i];
         __syncthreads();
