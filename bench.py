@@ -1,38 +1,52 @@
 """Benchmark: retro-edges per second of the plain k-cop solve on one MI355X.
 
-Workload (default): Erdos-Renyi G(200, 0.03), first connected seed >= 1
-(tests/golden/matrices/er200_p0.03_s1.txt), k = 3 cops, full information
-(p = 1): C(202, 3) = 1,353,400 configurations, 5.41e8 states. A step is one
-full solve with the tables resident in HBM: initializeCaptures + mainLoop
-(every depth level until the fixed point). The device tables are allocated
-once, before the timed region.
+Workload (default): planar200_k4 -- the Delaunay triangulation of 200 seeded
+uniform points in the unit square, thinned to 400 edges while connected
+(tests/golden/matrices/planar200_e400_s1.txt), k = 4 cops, full information
+(p = 1): C(203, 4) = 68,685,050 configurations, 2.75e10 states (27.5 GB
+table resident in HBM). Other workloads: er200_k3 (ER G(200, 0.03), first
+connected seed >= 1, k = 3, 5.41e8 states), the limited-visibility
+grid8x8_k2_p3, and batch4096. A step is one full solve with the tables
+resident in HBM: initializeCaptures + mainLoop (every depth level until the
+fixed point). The device tables are allocated once, before the timed region.
 
 Unit of work: a retro-edge, one predecessor relaxation of the push
 formulation, E_retro = E_cop + E_rob (won robber-turn states x |pred(C')| +
 won cop-turn states x in-degree(v), depth <= 125). value = E_retro * steps /
 elapsed, in GTEPS. The relaxations are counted from the solved table; how a
 level performs them (push, factorised pull, or the closed form of depth 1)
-does not change the count.
+does not change the count. `performed_gteps` leaves out the relaxations of
+depth-0 states, which the closed-form initialisation replaces.
 
 roofline: the dominant kernel's algorithmic bytes over its own HIP-event time
 (events on the solver's stream around each of its launches in the last timed
 step; a single-kernel launch such as the push records them itself through
 hipExtLaunchKernelGGL), against 8 TB/s. The dominant kernel and the per-kernel
 breakdown (kernels_ms_per_step) come from one untimed solve with every launch
-timed: a timed dispatch costs ~7 us more on a 4 ms ER solve, so only the last
-timed step carries events, and only around the dominant kernel.
+timed: a timed dispatch costs ~7 us more, so only the last timed step carries
+events, and only around the dominant kernel.
 Algorithmic bytes per unit (scalar-record model): 0.5 B per cop retro-edge
 (2-bit outcome read + written) for the cop-turn kernels of the levels they
 served, 2 B per robber retro-edge (8-bit escape counter read + written) for
 the robber step, 1.25 B per newly won cop-turn state (depth byte + frontier
-bit) for the finaliser.
+bit) for the finaliser. solve_roofline is the whole solve's algorithmic bytes
+(B_alg) over the step time.
 
 cpu_baseline: the reference solver (oracle/_ref/k_cops_visibility, built from
 the reference's own sources) on one host core, on a bounded sample of the
 workload -- the same graph with fewer cops, the most that keep the sample to
 at most 50,000 configurations (k = 2 for n = 200, ~2 s) -- as GTEPS of that
-run. cpu_port: the multithreaded level-synchronous CPU restatement of the
-oracle on the full workload, on the host's cores (skipped above 2e9 states).
+run. cpu_port: the oracle's multithreaded level-synchronous restatement on
+the host's cores. Up to 2e9 states it solves the whole workload and its
+table must equal the device's (the run fails otherwise); above that it runs
+under a time cap (--cpu-port-cap seconds) and reports the depth levels it
+finished and its rate over the relaxations it performed. reference_mt: the
+reference's own multithreaded solver (legacy/src/k_cops_4.cpp, stay-move
+rules, no depths) on the same graph at k = 3, wall time, for context only.
+
+batch (default line only): batch4096 -- one persistent launch over 4096 small
+graphs, the plain per-graph loop on a sample, and a Poisson request stream
+through the continuous-batching server (p50/p99 latency).
 """
 import argparse
 import json
@@ -108,14 +122,59 @@ def cpu_reference_sample(stem, k, e_retro):
                       f"{secs:.2f} s for {e_retro} retro-edges"}
 
 
-def cpu_port(stem, k, threads):
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def cpu_port(stem, k, threads, cap_s=None):
+    """The oracle's level-synchronous restatement with on-the-fly moves: the whole
+    workload (cap_s None), or until cap_s seconds have passed."""
     from oracle import binding as oracle
     n, adj = oracle.load_matrix(matrix_path(stem))
-    t0 = time.perf_counter()
-    states, _, e_cop, e_rob = oracle.v1_levelsync(adj, k, threads)
-    dt = time.perf_counter() - t0
-    return {"value": (e_cop + e_rob) / dt / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": f"multithreaded level-synchronous oracle, full workload, {dt:.2f} s"}, states
+    states, info = oracle.v1_levelsync_capped(adj, k, threads, 1e9 if cap_s is None else cap_s)
+    done = info["level_fraction"] == 0.0 and cap_s is None
+    rate = (info["e_cop"] + info["e_rob"]) / info["elapsed_s"] / 1e9
+    if cap_s is None:
+        sample = f"multithreaded level-synchronous oracle, full workload, {info['elapsed_s']:.2f} s"
+    else:
+        sample = (f"multithreaded level-synchronous oracle, full workload under a {cap_s:.0f} s cap: captures and "
+                  f"{info['levels_done']} depth levels done, {100 * info['level_fraction']:.0f} % of the next; "
+                  f"rate over the {info['e_cop'] + info['e_rob']} relaxations performed in {info['elapsed_s']:.1f} s "
+                  f"(capture initialisation included)")
+    return {"value": rate, "unit": "GTEPS", "cores": threads, "kind": "port", "sample": sample,
+            "levels_done": info["levels_done"], "complete": done}, states
+
+
+def reference_mt(stem, k, threads, timeout=180):
+    """The reference's multithreaded solver (legacy/src/k_cops_4.cpp, built by
+    oracle/build_ref.sh): stay-move rules and no depths, so a different
+    workload; wall time only, for context."""
+    exe = os.path.join(REPO, "oracle", "_ref", "k_cops_4")
+    if not os.path.exists(exe):
+        return None
+    work = tempfile.mkdtemp(prefix="crbench_")
+    try:
+        t0 = time.perf_counter()
+        proc = subprocess.run([exe, matrix_path(stem), str(k)], cwd=work, capture_output=True, text=True,
+                              timeout=timeout)
+        dt = time.perf_counter() - t0
+    except subprocess.TimeoutExpired:
+        return {"skipped": f"over {timeout} s"}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    m = re.search(r"RESULT: (WIN|LOSS)", proc.stdout)
+    return {"wall_s": dt, "threads": threads, "graph": stem, "k": k, "result": m.group(1) if m else None,
+            "note": "reference legacy/src/k_cops_4.cpp (hardware_concurrency threads, stay moves for cops and "
+                    "robber, win/loss only): a different rule set, timed for context, not a baseline"}
 
 
 def copy_calibration(nbytes):
@@ -240,13 +299,14 @@ def bench_macro(args):
     return 0
 
 
-def bench_batch(args):
+def run_batch(steps, warmup, plain_sample, serve_load):
     """4096 seeded connected graphs (n uniform in [8, 48], edge probability in
     [0.08, 0.3], k in {1, 2, 3}), full information. A step = one
     cr_batch_solve over all of them (one persistent launch; tables stay in
     HBM, verdicts come back), timed on the host clock around the call. Beside
     it: the plain six-step single-graph loop on a sample of the same graphs,
-    and a Poisson request stream through the continuous-batching server."""
+    and a Poisson request stream through the continuous-batching server.
+    Returns (block, graphs, batch results, sample indices)."""
     import threading
 
     import numpy as np
@@ -259,19 +319,20 @@ def bench_batch(args):
     graphs = workloads.batch_graphs(4096)
     states = sum(2 * __import__("math").comb(a.shape[0] + k - 1, k) * a.shape[0] for a, k in graphs)
     b = crb.Batch(0)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         ref = b.solve(graphs)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     dev_ms = 0.0
-    for _ in range(args.steps):
+    for _ in range(steps):
         ref = b.solve(graphs)
         dev_ms += b.last_timing()[0]
     elapsed = time.perf_counter() - t0
     grid = b.last_timing()[1]
-    batched = len(graphs) * args.steps / elapsed
+    b.close()
+    batched = len(graphs) * steps / elapsed
     # the plain loop: the six steps per graph on an evenly spaced sample
-    idx = np.linspace(0, len(graphs) - 1, min(args.plain_sample, len(graphs))).astype(int)
+    idx = np.linspace(0, len(graphs) - 1, min(plain_sample, len(graphs))).astype(int)
     s = cra.Solver(0)
     s.load_graph_matrix(graphs[0][0], graphs[0][1], 1)
     s.build_aux_graph()
@@ -287,7 +348,7 @@ def bench_batch(args):
     plain = len(idx) / (time.perf_counter() - t1)
     s.close()
     # serving: Poisson arrivals at a fraction of the batched rate
-    rate = args.serve_load * batched
+    rate = serve_load * batched
     rng = np.random.default_rng(12345)
     nreq = len(graphs)
     gaps = rng.exponential(1.0 / rate, nreq)
@@ -324,27 +385,47 @@ def bench_batch(args):
                 results[r["ticket"]] = r
         th.join()
         t_end = crb.clock_ns()
+        last_arrival = int(arrivals[-1])
         sstats = srv.stats()
         sstats["launches"] -= warm_launches
     lat = np.array([r["latency_ms"] for r in results.values()])
     mismatch = sum(1 for t, r in results.items() if r["table_hash"] != ref[tickets[t]]["table_hash"])
-    out = {
-        "metric": "batched graphs/s (4096 small graphs, one launch)", "value": batched, "unit": "graphs/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8/u64 bitsets (integer)",
-        "data": "synthetic (seeded graph generator)",
-        "config": {"workload": "batch4096", "graphs": len(graphs), "states": states, "workgroups": grid,
-                   "device_ms_per_step": dev_ms / args.steps},
+    table_bytes = states  # one DataItem byte per state, written once per solve
+    dev_s = dev_ms / 1e3 / steps
+    block = {
+        "graphs_per_s": batched, "ms_per_step": elapsed * 1e3 / steps, "device_ms_per_step": dev_ms / steps,
+        "graphs": len(graphs), "states": states, "workgroups": grid,
+        "roofline": {"bound": "hbm", "achieved": table_bytes / dev_s / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": table_bytes / dev_s / HBM_PEAK,
+                     "traffic": measured_traffic("batch4096", "k_batch_solve"), "kernel": "k_batch_solve",
+                     "alg_bytes_per_launch": table_bytes,
+                     "note": "algorithmic HBM bytes = the output tables (1 B per state); the planes of a "
+                             "graph live in LDS and its workgroup's L2-resident slot"},
         "plain_loop": {"graphs_per_s": plain, "sample": int(len(idx)), "speedup_batched": batched / plain},
-        "serving": {"arrival_rate_per_s": rate, "requests": nreq, "served_per_s": nreq / ((t_end - start) / 1e9),
+        "serving": {"arrival_rate_per_s": rate, "requests": nreq,
+                    "served_per_s": nreq / ((t_end - start) / 1e9),
+                    "drain_ms_after_last_arrival": (t_end - last_arrival) / 1e6,
                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                     "launches": sstats["launches"], "peak_admitted_bytes": sstats["peak_bytes"],
                     "results_differing_from_batch": mismatch},
-        "cpu_baseline": None,
+    }
+    return block, graphs, ref, idx
+
+
+def bench_batch(args):
+    block, graphs, ref, idx = run_batch(args.steps, args.warmup, args.plain_sample, args.serve_load)
+    out = {
+        "metric": "batched graphs/s (4096 small graphs, one launch)", "value": block["graphs_per_s"],
+        "unit": "graphs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": block["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8/u64 bitsets (integer)", "data": "synthetic (seeded graph generator)",
+        "config": {"workload": "batch4096", "graphs": block["graphs"], "states": block["states"],
+                   "workgroups": block["workgroups"], "device_ms_per_step": block["device_ms_per_step"]},
+        "roofline": block["roofline"], "plain_loop": block["plain_loop"], "serving": block["serving"],
+        "host": host_info(), "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_reference_batch(graphs, idx[:: max(1, len(idx) // 32)])
-    b.close()
     print(json.dumps(out))
     return 0
 
@@ -383,16 +464,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="er200_k3",
+    ap.add_argument("--workload", default="planar200_k4",
                     choices=sorted(WORKLOADS) + sorted(MACRO_WORKLOADS) + ["batch4096"])
     ap.add_argument("--plain-sample", type=int, default=256,
                     help="batch4096: graphs timed through the plain per-graph loop")
     ap.add_argument("--serve-load", type=float, default=0.7,
                     help="batch4096: Poisson arrival rate as a fraction of the batched throughput")
+    ap.add_argument("--no-batch", action="store_true", help="plain workloads: leave out the batch block")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-port", type=int, default=-1,
                     help="threads for the oracle-port timing (default: min(16, cores); 0 = skip)")
-    ap.add_argument("--check", action="store_true", help="compare the table with the oracle port")
+    ap.add_argument("--cpu-port-cap", type=float, default=20.0,
+                    help="seconds the CPU port may run on workloads above 2e9 states")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (rocprof counter calibration)")
     args = ap.parse_args()
@@ -403,14 +486,18 @@ def main():
                                    "design-only)"}))
         return 2
 
-    import torch
-
-    import copsandrobbers_amd as cra
-
     if args.workload in MACRO_WORKLOADS:
         return bench_macro(args)
     if args.workload == "batch4096":
         return bench_batch(args)
+    return bench_plain(args)
+
+
+def bench_plain(args):
+    import torch
+
+    import copsandrobbers_amd as cra
+
     stem, k = WORKLOADS[args.workload]
     path = matrix_path(stem)
     s = cra.Solver(0)
@@ -427,8 +514,6 @@ def main():
     s.main_loop()
     breakdown = {name: ms for name, (ms, _) in s.kernel_times().items()}
     dominant = max(breakdown, key=breakdown.get)
-    # the last timed step carries HIP events around the dominant kernel's
-    # launches (~7 us per timed dispatch on a 4 ms ER solve, so not every step)
     s.set_kernel_timing(False)
     kt_total = {}
     torch.cuda.synchronize()
@@ -473,6 +558,7 @@ def main():
               "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
               "launches_per_step": launches}
     b_alg = 2 * e_rob + 0.5 * e_cop + won * 1.25 + info.num_states * (0.25 + 1) + info.num_states / 2
+    performed = e_retro - int(e_cop_d[0]) - int(e_rob_d[0])
     out = {
         "metric": "retro-edges/s (GTEPS)",
         "value": value,
@@ -489,11 +575,17 @@ def main():
         "config": {"workload": args.workload, "graph": stem, "n": info.n, "k": k, "p": 1,
                    "configs": info.config_count, "states": info.num_states,
                    "e_cop": e_cop, "e_rob": e_rob, "won_states": won, "levels": stats.n_levels,
-                   "passes": len(passes), "verdict": "WIN" if verdict else "LOSS"},
+                   "passes": len(passes), "verdict": "WIN" if verdict else "LOSS",
+                   "positions": list(verdict["positions"]) if verdict else None,
+                   "plys": verdict["plys"] if verdict else None},
+        "performed_gteps": {"value": performed * args.steps / elapsed / 1e9, "relaxations": performed,
+                            "note": "relaxations from states of depth >= 1; those of the depth-0 captures are "
+                                    "replaced by the closed-form initialisation"},
         "roofline": rl,
         "solve_roofline": {"alg_bytes": b_alg, "achieved_GBps": b_alg / (elapsed / args.steps) / 1e9,
                            "frac": b_alg / (elapsed / args.steps) / HBM_PEAK},
         "kernels_ms_per_step": breakdown,
+        "host": host_info(),
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
@@ -512,20 +604,26 @@ def main():
             se = sum(t.retro_edges()[:2])
         out["cpu_baseline"] = cpu_reference_sample(stem, sk, se)
     if args.cpu_port < 0:
-        # the port solves the full workload in host memory: up to 2e9 states
-        too_big = info.num_states > 2_000_000_000
-        args.cpu_port = 0 if args.no_cpu_baseline or too_big else min(16, os.cpu_count() or 1)
-        if too_big and not args.no_cpu_baseline:
-            out["cpu_port"] = {"skipped": f"{info.num_states} states: the full workload does not fit the "
-                                          "CPU port's host tables within minutes"}
+        args.cpu_port = 0 if args.no_cpu_baseline else min(16, len(os.sched_getaffinity(0)))
+    mismatch = False
     if args.cpu_port:
-        port, states = cpu_port(stem, k, args.cpu_port)
+        capped = info.num_states > 2_000_000_000
+        port, states = cpu_port(stem, k, args.cpu_port, args.cpu_port_cap if capped else None)
         out["cpu_port"] = port
-        if args.check:
+        if not capped:
             import numpy as np
-            out["check_vs_port"] = bool(np.array_equal(states, s.states()))
+            same = bool(np.array_equal(states, s.states()))
+            out["check_vs_port"] = same
+            mismatch = not same
+        del states
+        out["reference_mt"] = reference_mt(stem, 3, args.cpu_port)
     s.close()
+    if args.workload == "planar200_k4" and not args.no_batch:
+        out["batch"] = run_batch(5, 2, 128, 0.7)[0]
     print(json.dumps(out))
+    if mismatch:
+        print("bench: the device table differs from the CPU port's", file=sys.stderr)
+        return 1
     return 0
 
 

@@ -52,6 +52,16 @@ def lib():
         L.or_retro_edges.restype = i32
         L.or_pass_counts_from_depths.argtypes = [i32, i32, u8p, u64p, i32]
         L.or_pass_counts_from_depths.restype = i32
+        L.or_unrank.argtypes = [u64, i32, i32, u8p]
+        L.or_unrank.restype = i32
+        L.or_check_states.argtypes = [i32, u8p, i32, u8p, u64p, u64, i32, u64p, u64p]
+        L.or_check_states.restype = i32
+        L.or_table_summary.argtypes = [i32, i32, u8p, i32, u64p, ctypes.POINTER(i64), ctypes.POINTER(i32)]
+        L.or_table_summary.restype = i32
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.or_v1_levelsync_capped.argtypes = [i32, u8p, i32, i32, ctypes.c_double, u8p, u64p, u64p,
+                                             ctypes.POINTER(i32), dp, dp]
+        L.or_v1_levelsync_capped.restype = i32
         _lib = L
     return _lib
 
@@ -183,6 +193,73 @@ def retro_edges(adj, k, states):
     assert lib().or_retro_edges(adj.shape[0], _p8(adj), k, _p8(states), ctypes.byref(ec),
                                 ctypes.byref(er), ctypes.byref(w)) == 0
     return int(ec.value), int(er.value), int(w.value)
+
+
+def unrank(rank_, n, k):
+    out = np.zeros(k, np.uint8)
+    if lib().or_unrank(rank_, n, k, _p8(out)) != 0:
+        raise ValueError("rank out of range")
+    return tuple(int(x) for x in out)
+
+
+def check_states(adj, k, states, ids, nthreads=1):
+    """Bellman check of a solved p = 1 table at the state indices `ids`.
+    Returns (number of states that differ, index of one of them or None)."""
+    adj = np.ascontiguousarray(adj, np.uint8)
+    ids = np.ascontiguousarray(ids, np.uint64)
+    bad, first = ctypes.c_uint64(), ctypes.c_uint64()
+    rc = lib().or_check_states(adj.shape[0], _p8(adj), k, _p8(states), _p64(ids), len(ids), nthreads,
+                               ctypes.byref(bad), ctypes.byref(first))
+    if rc != 0:
+        raise RuntimeError("or_check_states failed")
+    return int(bad.value), (int(first.value) if bad.value else None)
+
+
+def table_summary(n, k, states, nthreads=1):
+    """(level_counts[127, 2], best_cid or -1, plys or -1) of a solved p = 1 table."""
+    lv = np.zeros(2 * MAX_ROUND, np.uint64)
+    best, plys = ctypes.c_int64(), ctypes.c_int32()
+    if lib().or_table_summary(n, k, _p8(states), nthreads, _p64(lv), ctypes.byref(best),
+                              ctypes.byref(plys)) != 0:
+        raise RuntimeError("or_table_summary failed")
+    return lv.reshape(MAX_ROUND, 2), int(best.value), int(plys.value)
+
+
+def v1_levelsync_capped(adj, k, nthreads=1, time_cap_s=1e9, states=None):
+    """Time-capped level-synchronous solve with on-the-fly moves. Returns
+    (states, dict(e_cop, e_rob, levels_done, level_fraction, elapsed_s))."""
+    adj = np.ascontiguousarray(adj, np.uint8)
+    n = adj.shape[0]
+    if states is None:
+        states = np.empty(config_count(n, k) * n * 2, np.uint8)
+    ec, er = ctypes.c_uint64(), ctypes.c_uint64()
+    done, frac, el = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+    rc = lib().or_v1_levelsync_capped(n, _p8(adj), k, nthreads, float(time_cap_s), _p8(states),
+                                      ctypes.byref(ec), ctypes.byref(er), ctypes.byref(done),
+                                      ctypes.byref(frac), ctypes.byref(el))
+    if rc != 0:
+        raise RuntimeError("or_v1_levelsync_capped failed")
+    return states, {"e_cop": int(ec.value), "e_rob": int(er.value), "levels_done": int(done.value),
+                    "level_fraction": float(frac.value), "elapsed_s": float(el.value)}
+
+
+def passes_from_levels(levels):
+    """The per-pass counts of the p = 1 sweep from a (round, turn) histogram of
+    marked states: pass i prints the cop-turn states of rounds 2i-2 (above 0)
+    and 2i-1 plus the robber-turn states of rounds 2i-1 and 2i, and the sweep
+    stops after the first pass that finds nothing (the rule of
+    or_pass_counts_from_depths, applied to counts instead of a table)."""
+    lv = np.asarray(levels, np.uint64).reshape(MAX_ROUND, 2)
+    get = lambda d, t: int(lv[d, t]) if 0 <= d < MAX_ROUND else 0
+    out = []
+    i = 1
+    while True:
+        c0 = 2 * i - 2
+        cnt = (get(c0, 0) if c0 > 0 else 0) + get(2 * i - 1, 0) + get(2 * i - 1, 1) + get(2 * i, 1)
+        out.append(cnt)
+        if cnt == 0:
+            return out
+        i += 1
 
 
 def pass_counts_from_depths(n, k, states, max_passes=8192):

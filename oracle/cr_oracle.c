@@ -23,11 +23,20 @@
  *   or_retro_edges      brute-force recount in Python on small cases and
  *                       equality with or_v1_levelsync's own counters.
  *   or_pass_counts_from_depths  equal to or_v1_solve's printed pass counts.
+ *   or_unrank           inverse of or_rank over every tuple, N<=11, k<=4.
+ *   or_check_states     0 differences on every golden p = 1 table (all states);
+ *                       planted one-state mistakes (a depth off by one, a
+ *                       marked state cleared, an unmarked state marked) found.
+ *   or_table_summary    = or_verdict and or_v1_levelsync level counts on the
+ *                       golden cases.
+ *   or_v1_levelsync_capped  uncapped: byte-equal to or_v1_levelsync with equal
+ *                       relaxation counts on the golden undirected cases.
  */
 #define _GNU_SOURCE
 #include "cr_oracle.h"
 
 #include <pthread.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -812,4 +821,349 @@ int32_t or_pass_counts_from_depths(int32_t n, int32_t k, const uint8_t* states,
         if (cnt == 0) break;
     }
     return passes;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Checks of a solved p = 1 table at sizes the sweep cannot reach            */
+/* ------------------------------------------------------------------------ */
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* Inverse of or_rank: the non-decreasing tuple at lexicographic position
+ * `rank` (AuxGraph.h:155-176 enumeration order). A prefix ending in value v
+ * at position i is followed by C(N-v+k-i-2, k-i-1) completions. */
+int32_t or_unrank(uint64_t rank, int32_t n, int32_t k, uint8_t* out) {
+    if (k <= 0 || k > OR_MAX_K || n <= 0 || rank >= or_config_count(n, k)) return -1;
+    int v = 0;
+    for (int i = 0; i < k; ++i) {
+        for (;; ++v) {
+            uint64_t span = binom((int64_t)n - v + k - i - 2, k - i - 1);
+            if (rank < span) break;
+            rank -= span;
+        }
+        out[i] = (uint8_t)v;
+    }
+    return 0;
+}
+
+/* The value a state must hold given its successors' values in the same table:
+ * the local (Bellman) form of solvers/k_cops_visibility.cpp:85-299 at p = 1.
+ *   capture (r in C, both turns, :98-110): marked at round 0;
+ *   cop turn (:160-197): 1 + min over team moves C' in T(C) of the marked
+ *     robber-turn rounds of (C', r);
+ *   robber turn (:235-288): 1 + max over r -> v of the cop-turn rounds of
+ *     (C, v) when all of them are marked (vacuous for a robber with no move);
+ * and a state is marked only below the cap (round + 1 < 127, :189, :280).
+ * Applied at every state, these equations have exactly one solution (by
+ * induction on the round: a state holds round d iff its true depth is d), so
+ * a table that satisfies them everywhere is the reference's table. */
+static uint8_t expected_byte(const nbrs_t* g, int32_t k, const uint8_t* states,
+                             const uint8_t* cops, uint64_t c, int r, int t, uint64_t* buf,
+                             uint64_t cap) {
+    const uint64_t N = (uint64_t)g->n;
+    if (in_config(cops, k, r)) return WON_BYTE(0);
+    unsigned best;
+    if (t == 0) {
+        int64_t m = moves_of(g, cops, k, 0, buf, cap);
+        best = OR_MAX_ROUND;
+        for (int64_t i = 0; i < m; ++i) {
+            uint8_t b = states[(buf[i] * N + (uint64_t)r) * 2 + 1];
+            if (IS_MARKED(b) && DEPTH_OF(b) < best) best = DEPTH_OF(b);
+        }
+        if (best == OR_MAX_ROUND) return UNMARKED_BYTE;
+    } else {
+        best = 0;
+        for (int e = 0; e < g->deg[r]; ++e) {
+            uint8_t b = states[(c * N + g->nb[r][e]) * 2];
+            if (!IS_MARKED(b)) return UNMARKED_BYTE;
+            if (DEPTH_OF(b) > best) best = DEPTH_OF(b);
+        }
+    }
+    return best + 1 < OR_MAX_ROUND ? WON_BYTE(best + 1) : UNMARKED_BYTE;
+}
+
+typedef struct {
+    const nbrs_t* g;
+    int32_t k;
+    const uint8_t* states;
+    const uint64_t* ids;
+    uint64_t lo, hi;
+    uint64_t bad, first_bad;
+    int err;
+} chk_job;
+
+static void* chk_run(void* arg) {
+    chk_job* j = (chk_job*)arg;
+    const uint64_t N = (uint64_t)j->g->n;
+    uint64_t cap = product_bound(j->g, j->k, 0);
+    uint64_t* buf = (uint64_t*)malloc(sizeof(uint64_t) * (cap ? cap : 1));
+    if (!buf) { j->err = 1; return NULL; }
+    uint8_t cops[OR_MAX_K];
+    uint64_t last_c = UINT64_MAX;
+    for (uint64_t i = j->lo; i < j->hi; ++i) {
+        uint64_t id = j->ids[i], t = id & 1, c = (id >> 1) / N;
+        int r = (int)((id >> 1) % N);
+        if (c != last_c) { or_unrank(c, j->g->n, j->k, cops); last_c = c; }
+        uint8_t want = expected_byte(j->g, j->k, j->states, cops, c, r, (int)t, buf, cap);
+        if (j->states[id] != want) {
+            if (j->bad++ == 0) j->first_bad = id;
+        }
+    }
+    free(buf);
+    return NULL;
+}
+
+int32_t or_check_states(int32_t n, const uint8_t* adj, int32_t k, const uint8_t* states,
+                        const uint64_t* ids, uint64_t count, int32_t nthreads, uint64_t* bad,
+                        uint64_t* first_bad) {
+    if (n <= 0 || n > 255 || k <= 0 || k > OR_MAX_K || nthreads < 1 || nthreads > 1024) return -1;
+    uint64_t S = or_config_count(n, k) * (uint64_t)n * 2;
+    for (uint64_t i = 0; i < count; ++i) if (ids[i] >= S) return -1;
+    nbrs_t* g = (nbrs_t*)malloc(sizeof(nbrs_t));
+    chk_job* jobs = (chk_job*)calloc((size_t)nthreads, sizeof(chk_job));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    if (!g || !jobs || !th) { free(g); free(jobs); free(th); return -1; }
+    build_nbrs(g, n, adj);
+    for (int t = 0; t < nthreads; ++t) {
+        jobs[t] = (chk_job){g, k, states, ids, count * (uint64_t)t / (uint64_t)nthreads,
+                            count * (uint64_t)(t + 1) / (uint64_t)nthreads, 0, 0, 0};
+        pthread_create(&th[t], NULL, chk_run, &jobs[t]);
+    }
+    uint64_t nb = 0, fb = UINT64_MAX;
+    int err = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        pthread_join(th[t], NULL);
+        err |= jobs[t].err;
+        if (jobs[t].bad && nb == 0) fb = jobs[t].first_bad;
+        nb += jobs[t].bad;
+    }
+    free(g); free(jobs); free(th);
+    *bad = nb;
+    *first_bad = fb;
+    return err ? -1 : 0;
+}
+
+/* One pass over a p = 1 table: histogram of marked states by (round, turn)
+ * and the verdict of findFinalResult (k_cops_visibility.cpp:301-354: the
+ * configuration whose column-0 states are all marked with the least worst
+ * round, lowest cId on ties). Rows are split over threads; each thread keeps
+ * its own best, and the lowest cId wins among equal worst rounds. */
+typedef struct {
+    int32_t n;
+    const uint8_t* states;
+    uint64_t lo, hi;
+    uint64_t lv[2][OR_MAX_ROUND];
+    int64_t best;
+    unsigned best_worst;
+} sum_job;
+
+static void* sum_run(void* arg) {
+    sum_job* j = (sum_job*)arg;
+    const uint64_t N = (uint64_t)j->n;
+    j->best = -1;
+    j->best_worst = OR_MAX_ROUND;
+    for (uint64_t c = j->lo; c < j->hi; ++c) {
+        const uint8_t* row = j->states + c * N * 2;
+        unsigned worst = 0;
+        int all = 1;
+        for (uint64_t r = 0; r < N; ++r) {
+            uint8_t b0 = row[2 * r], b1 = row[2 * r + 1];
+            if (IS_MARKED(b0)) { j->lv[0][DEPTH_OF(b0)]++; if (DEPTH_OF(b0) > worst) worst = DEPTH_OF(b0); }
+            else all = 0;
+            if (IS_MARKED(b1)) j->lv[1][DEPTH_OF(b1)]++;
+        }
+        if (all && worst < j->best_worst) { j->best_worst = worst; j->best = (int64_t)c; }
+    }
+    return NULL;
+}
+
+int32_t or_table_summary(int32_t n, int32_t k, const uint8_t* states, int32_t nthreads,
+                         uint64_t* level_counts, int64_t* best_cid, int32_t* plys) {
+    if (n <= 0 || n > 255 || nthreads < 1 || nthreads > 1024) return -1;
+    uint64_t C = or_config_count(n, k);
+    sum_job* jobs = (sum_job*)calloc((size_t)nthreads, sizeof(sum_job));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    if (!jobs || !th) { free(jobs); free(th); return -1; }
+    for (int t = 0; t < nthreads; ++t) {
+        jobs[t].n = n;
+        jobs[t].states = states;
+        jobs[t].lo = C * (uint64_t)t / (uint64_t)nthreads;
+        jobs[t].hi = C * (uint64_t)(t + 1) / (uint64_t)nthreads;
+        pthread_create(&th[t], NULL, sum_run, &jobs[t]);
+    }
+    memset(level_counts, 0, sizeof(uint64_t) * 2 * OR_MAX_ROUND);
+    int64_t best = -1;
+    unsigned best_worst = OR_MAX_ROUND;
+    for (int t = 0; t < nthreads; ++t) { /* threads in cId order: strict < keeps the lowest */
+        pthread_join(th[t], NULL);
+        for (int d = 0; d < OR_MAX_ROUND; ++d) {
+            level_counts[2 * d] += jobs[t].lv[0][d];
+            level_counts[2 * d + 1] += jobs[t].lv[1][d];
+        }
+        if (jobs[t].best >= 0 && jobs[t].best_worst < best_worst) {
+            best_worst = jobs[t].best_worst;
+            best = jobs[t].best;
+        }
+    }
+    free(jobs); free(th);
+    *best_cid = best;
+    *plys = best >= 0 ? (int32_t)best_worst : -1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Level-synchronous p = 1 solver with moves generated on the fly and a time */
+/* cap (the CPU baseline at sizes where a move table does not fit).          */
+/* Structure of legacy/src/k_cops_4.cpp:388-481 (threads over the frontier,  */
+/* atomic claims and escape counters), v1 rules of k_cops_visibility.cpp.    */
+/* Undirected graphs only: the predecessors of (C', r, 1) are then           */
+/* {(C, r, 0) : C in T(C')}. The frontier of depth d is found by scanning    */
+/* the table for that depth instead of being kept in lists.                  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const nbrs_t* g;
+    int32_t k;
+    uint64_t C;
+    uint8_t* states;
+    uint8_t* escapes;
+    unsigned depth;
+    double deadline;
+    volatile int* stop;
+    uint64_t* next_row; /* shared chunk counter */
+    int phase;          /* 0 = init, 1 = level */
+} cap_shared;
+
+typedef struct {
+    cap_shared* sh;
+    uint64_t e_cop, e_rob, rows;
+    int err;
+} cap_worker;
+
+#define CAP_CHUNK 256
+
+static void* cap_run(void* arg) {
+    cap_worker* w = (cap_worker*)arg;
+    cap_shared* s = w->sh;
+    const nbrs_t* g = s->g;
+    const uint64_t N = (uint64_t)g->n;
+    uint64_t cap = product_bound(g, s->k, 0);
+    uint64_t* buf = (uint64_t*)malloc(sizeof(uint64_t) * (cap ? cap : 1));
+    if (!buf) { w->err = 1; return NULL; }
+    uint8_t cops[OR_MAX_K];
+    const uint8_t nd = WON_BYTE(s->depth + 1);
+    for (;;) {
+        if (*s->stop) break;
+        uint64_t c0 = __atomic_fetch_add(s->next_row, CAP_CHUNK, __ATOMIC_RELAXED);
+        if (c0 >= s->C) break;
+        uint64_t c1 = c0 + CAP_CHUNK < s->C ? c0 + CAP_CHUNK : s->C;
+        for (uint64_t c = c0; c < c1; ++c) {
+            or_unrank(c, g->n, s->k, cops);
+            uint8_t* row = s->states + c * N * 2;
+            if (s->phase == 0) {
+                for (uint64_t r = 0; r < N; ++r) {
+                    uint8_t b = in_config(cops, s->k, (int)r) ? WON_BYTE(0) : UNMARKED_BYTE;
+                    row[2 * r] = b;
+                    row[2 * r + 1] = b;
+                    s->escapes[c * N + r] = (uint8_t)g->deg[r];
+                }
+                continue;
+            }
+            int moves_known = 0;
+            int64_t m = 0;
+            for (uint64_t r = 0; r < N; ++r) {
+                /* robber-turn state of this depth -> cop-turn predecessors */
+                uint8_t b1 = __atomic_load_n(&row[2 * r + 1], __ATOMIC_RELAXED);
+                if (b1 == WON_BYTE(s->depth)) {
+                    if (!moves_known) { m = moves_of(g, cops, s->k, 0, buf, cap); moves_known = 1; }
+                    for (int64_t i = 0; i < m; ++i) {
+                        uint8_t* b = &s->states[(buf[i] * N + r) * 2];
+                        uint8_t expect = UNMARKED_BYTE;
+                        w->e_cop++;
+                        if (__atomic_load_n(b, __ATOMIC_RELAXED) != UNMARKED_BYTE) continue;
+                        __atomic_compare_exchange_n(b, &expect, nd, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED);
+                    }
+                }
+                /* cop-turn state of this depth -> robber-turn predecessors */
+                uint8_t b0 = __atomic_load_n(&row[2 * r], __ATOMIC_RELAXED);
+                if (b0 == WON_BYTE(s->depth)) {
+                    for (int e = 0; e < g->deg[r]; ++e) {
+                        uint64_t u = g->nb[r][e]; /* undirected: in-neighbours = neighbours */
+                        w->e_rob++;
+                        uint8_t* b = &row[2 * u + 1];
+                        if (IS_MARKED(__atomic_load_n(b, __ATOMIC_RELAXED))) continue;
+                        if (__atomic_fetch_sub(&s->escapes[c * N + u], 1, __ATOMIC_RELAXED) == 1)
+                            __atomic_store_n(b, nd, __ATOMIC_RELAXED);
+                    }
+                }
+            }
+        }
+        w->rows += c1 - c0;
+        if (now_s() > s->deadline) *s->stop = 1;
+    }
+    free(buf);
+    return NULL;
+}
+
+int32_t or_v1_levelsync_capped(int32_t n, const uint8_t* adj, int32_t k, int32_t nthreads,
+                               double time_cap_s, uint8_t* states, uint64_t* e_cop,
+                               uint64_t* e_rob, int32_t* levels_done, double* level_fraction,
+                               double* elapsed_s) {
+    if (n <= 0 || n > 255 || k <= 0 || k > OR_MAX_K || nthreads < 1 || nthreads > 1024) return -1;
+    for (int u = 0; u < n; ++u)
+        for (int v = 0; v < n; ++v)
+            if (adj[(size_t)u * n + v] != adj[(size_t)v * n + u]) return -1;
+    nbrs_t* g = (nbrs_t*)malloc(sizeof(nbrs_t));
+    if (!g) return -1;
+    build_nbrs(g, n, adj);
+    for (int u = 0; u < n; ++u) if (g->deg[u] == 0) { free(g); return -1; }
+    uint64_t C = or_config_count(n, k);
+    uint8_t* esc = (uint8_t*)malloc(C * (uint64_t)n);
+    cap_worker* ws = (cap_worker*)calloc((size_t)nthreads, sizeof(cap_worker));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    if (!esc || !ws || !th) { free(g); free(esc); free(ws); free(th); return -1; }
+    double t0 = now_s();
+    volatile int stop = 0;
+    uint64_t ec = 0, er = 0;
+    int done = 0, err = 0;
+    double frac = 0.0;
+    /* phase 0 = the captures (initializeCaptures, :85-127), then one phase
+     * per depth until a level marks nothing or the cap stops it */
+    for (int phase = 0; !stop; ++phase) {
+        unsigned depth = phase == 0 ? 0 : (unsigned)(phase - 1);
+        if (phase > 0 && depth + 1 >= OR_MAX_ROUND) break;
+        uint64_t next = 0;
+        cap_shared sh = {g, k, C, states, esc, depth, t0 + time_cap_s, &stop, &next, phase == 0 ? 0 : 1};
+        for (int t = 0; t < nthreads; ++t) {
+            memset(&ws[t], 0, sizeof(cap_worker));
+            ws[t].sh = &sh;
+            pthread_create(&th[t], NULL, cap_run, &ws[t]);
+        }
+        uint64_t rows = 0, lec = 0, ler = 0;
+        for (int t = 0; t < nthreads; ++t) {
+            pthread_join(th[t], NULL);
+            err |= ws[t].err;
+            rows += ws[t].rows;
+            lec += ws[t].e_cop;
+            ler += ws[t].e_rob;
+        }
+        ec += lec;
+        er += ler;
+        if (phase == 0) {
+            if (rows < C) { frac = 0.0; break; } /* the cap ended the captures */
+            continue;
+        }
+        if (rows < C) { frac = (double)rows / (double)C; break; }
+        done++;
+        if (lec == 0 && ler == 0) break; /* no state of this depth: fixed point */
+    }
+    if (elapsed_s) *elapsed_s = now_s() - t0;
+    free(g); free(esc); free(ws); free(th);
+    *e_cop = ec;
+    *e_rob = er;
+    *levels_done = done;
+    *level_fraction = frac;
+    return err ? -1 : 0;
 }

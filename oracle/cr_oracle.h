@@ -84,6 +84,36 @@ int32_t or_retro_edges(int32_t n, const uint8_t* adj, int32_t k, const uint8_t* 
 int32_t or_pass_counts_from_depths(int32_t n, int32_t k, const uint8_t* states,
                                    uint64_t* pass_counts, int32_t max_passes);
 
+/* Inverse of or_rank: the tuple at lexicographic position rank. */
+int32_t or_unrank(uint64_t rank, int32_t n, int32_t k, uint8_t* out);
+
+/* Local (Bellman) check of a solved p = 1 table, plain rules, at the given
+ * state indices ((cId*N + r)*2 + turn): each state must hold the byte its
+ * successors in the same table imply (captures at round 0; cop turn 1 + min
+ * over T(C), robber turn 1 + max over N(r) when all marked; cap 127), per
+ * solvers/k_cops_visibility.cpp:85-299. bad = number of states that differ,
+ * first_bad = the index of one of them (UINT64_MAX if none). */
+int32_t or_check_states(int32_t n, const uint8_t* adj, int32_t k, const uint8_t* states,
+                        const uint64_t* ids, uint64_t count, int32_t nthreads, uint64_t* bad,
+                        uint64_t* first_bad);
+
+/* One multithreaded pass over a p = 1 table: level_counts[2*d+t] = marked
+ * states of turn t and round d, and the findFinalResult verdict
+ * (k_cops_visibility.cpp:301-354; best_cid = -1 for LOSS). */
+int32_t or_table_summary(int32_t n, int32_t k, const uint8_t* states, int32_t nthreads,
+                         uint64_t* level_counts, int64_t* best_cid, int32_t* plys);
+
+/* Level-synchronous p = 1 solver (undirected graphs, plain rules) with team
+ * moves generated on the fly and a wall-clock cap: the CPU baseline where a
+ * move table does not fit. Same table as or_v1_levelsync when it finishes.
+ * levels_done = depth levels completed; level_fraction = share of the
+ * configurations the interrupted level had scanned; e_cop/e_rob = the
+ * relaxations performed until then; elapsed_s includes the captures. */
+int32_t or_v1_levelsync_capped(int32_t n, const uint8_t* adj, int32_t k, int32_t nthreads,
+                               double time_cap_s, uint8_t* states, uint64_t* e_cop,
+                               uint64_t* e_rob, int32_t* levels_done, double* level_fraction,
+                               double* elapsed_s);
+
 #ifdef __cplusplus
 }
 #endif

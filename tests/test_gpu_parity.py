@@ -219,11 +219,17 @@ PULL_SETTINGS = [{"pull_scale": 0, "pull_tiles": 1, "pull_tile_rows": 4},
                  {"pull_scale": 0, "depth_log": 40, "depth_log_min": 0},
                  {"pull_scale": 0, "depth_log": 40, "depth_log_min": 0, "fuse_finalize": 0},
                  {"depth_log": 40, "depth_log_min": 0},
-                 {"pull_scale": 0, "depth_log": 2, "depth_log_min": 0, "levels_per_sync": 3}]
+                 {"pull_scale": 0, "depth_log": 2, "depth_log_min": 0, "levels_per_sync": 3},
+                 {"pred_csr": 1, "push_dirty": 0, "pull_scale": 10**9},
+                 {"pred_csr": 1, "push_dirty": 1, "pull_scale": 10**9},
+                 {"push_filter": 0, "pull_scale": 10**9},
+                 {"push_blocks": 8, "pull_scale": 10**9}]
 PULL_IDS = ["pull_tiles4", "pull_tiles3_chunk5", "pull_tile1_off", "pull_tile1_3", "pull_gate1", "robber_by_new", "robber_by_open", "small_grids", "scan_tiles4", "push_no_marks", "pull_rows", "push_only", "direct_bytes", "pull_log_fused",
-            "pull_log_unfused", "depth_log", "small_log"]
+            "pull_log_unfused", "depth_log", "small_log", "push_csr", "push_csr_marks", "push_unfiltered",
+            "push_8_blocks"]
 DEFAULTS = {"pull_scale": -1, "pull_tiles": 1, "pull_tile_rows": 4, "pull_tile_chunk": 16, "pull_tile_blocks": 8192, "depth_log": 40, "depth_log_min": 1 << 24,
-            "levels_per_sync": 8, "fuse_finalize": 1, "pull_tile1": 9, "pull_gate": 4, "robber_open": 2, "row_blocks": 32768, "log_blocks": 4096, "scan_tile": 8192, "push_dirty": 1}
+            "levels_per_sync": 8, "fuse_finalize": 1, "pull_tile1": 9, "pull_gate": 4, "robber_open": 2, "row_blocks": 32768, "log_blocks": 4096, "scan_tile": 8192, "push_dirty": 1,
+            "pred_csr": 0, "push_filter": 1, "push_blocks": 2048}
 PULL_CASES = ["v1__cycle100__k2__p1", "v1__dodecahedron__k3__p1", "v1__grid36__k3__p1", "v1__grid9__k3__p1",
               "v1__line3__k2__p1", "v1__outerplanar8__k3__p1", "v1__peterson__k3__p1", "v1__planar32__k3__p1",
               "v1__scotlandyard-all__k2__p1", "v1__test__k2__p1"]

@@ -1,7 +1,8 @@
 """Randomised parity of the plain solve (v1, p = 1) with the oracle: random small
-graphs (6..40 vertices, 1..4 cops) under random speed settings, none of which
-may change a table. Fixed seeds; every case prints its graph size and settings
-before it runs."""
+graphs (6..40 vertices, 1..4 cops), and graphs of 65..130 vertices with 1..2
+cops whose robber rows take two or three 64-bit words, under random speed
+settings, none of which may change a table. Fixed seeds; every case prints its
+graph size and settings before it runs."""
 import numpy as np
 import pytest
 
@@ -23,13 +24,16 @@ CHOICES = {
     "levels_per_sync": [1, 2, 3, 5, 8],
     "scan_tile": [4, 7, 8192],
     "push_dirty": [0, 1],
+    "pred_csr": [0, 0, 1],
+    "push_filter": [0, 1, 1],
+    "push_blocks": [8, 2048],
 }
 
 
-def random_graph(rng):
+def random_graph(rng, lo=6, hi=40):
     """A random undirected graph in which every vertex has a move."""
     while True:
-        n = int(rng.integers(6, 41))
+        n = int(rng.integers(lo, hi + 1))
         adj = (rng.random((n, n)) < rng.uniform(0.06, 0.35)).astype(np.uint8)
         adj = np.triu(adj, 1)
         adj = adj | adj.T
@@ -53,6 +57,27 @@ def test_random_graphs_and_settings_match_oracle(oracle, seed):
             s.initialize_captures()
             s.main_loop()
             want = oracle.v1_solve(adj, k, 1).states
+            assert np.array_equal(s.states(), want), (seed, case, n, k, tuning)
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("seed", [21])
+def test_random_wide_rows_and_settings_match_oracle(oracle, seed):
+    rng = np.random.default_rng(seed)
+    s = cra.Solver(0)
+    try:
+        for case in range(12):
+            n, adj = random_graph(rng, 65, 130)
+            k = int(rng.integers(1, 3))
+            tuning = {name: int(rng.choice(values)) for name, values in CHOICES.items()}
+            print(f"random wide case {seed}/{case}: n={n} k={k} {tuning}", flush=True)
+            s.set_tuning(**tuning)
+            s.load_graph_matrix(adj, k, 1)
+            s.build_aux_graph()
+            s.initialize_captures()
+            s.main_loop()
+            want = oracle.v1_levelsync(adj, k, nthreads=8)[0]
             assert np.array_equal(s.states(), want), (seed, case, n, k, tuning)
     finally:
         s.close()
