@@ -122,6 +122,11 @@ def cpu_reference_sample(stem, k, e_retro):
                       f"{secs:.2f} s for {e_retro} retro-edges"}
 
 
+def progress(*args):
+    """One line per finished phase on stderr (stdout carries only the JSON line)."""
+    print("bench:", *args, file=sys.stderr, flush=True)
+
+
 def host_info():
     model = None
     try:
@@ -154,7 +159,7 @@ def cpu_port(stem, k, threads, cap_s=None):
             "levels_done": info["levels_done"], "complete": done}, states
 
 
-def reference_mt(stem, k, threads, timeout=180):
+def reference_mt(stem, k, threads, timeout=100):
     """The reference's multithreaded solver (legacy/src/k_cops_4.cpp, built by
     oracle/build_ref.sh): stay-move rules and no depths, so a different
     workload; wall time only, for context."""
@@ -504,9 +509,11 @@ def bench_plain(args):
     s.load_graph_file(path, k, 1)
     s.build_aux_graph()
     info = s.info()
+    progress(f"{args.workload}: {info.num_states} states, device buffers {info.device_bytes} B")
     for _ in range(args.warmup):
         s.initialize_captures()
         s.main_loop()
+    progress("warm-up done")
     # per-kernel breakdown from one untimed solve with every launch timed; the
     # timed steps then carry events around the dominant kernel's launches only
     s.set_kernel_timing(True)
@@ -525,6 +532,7 @@ def bench_plain(args):
         passes = s.main_loop()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    progress(f"timed {args.steps} steps: {elapsed * 1e3 / args.steps:.3f} ms per step")
     for name, (ms, launches) in s.kernel_times().items():  # the last timed step
         kt_total[name] = [ms, launches]
     s.set_kernel_timing(False)
@@ -603,6 +611,7 @@ def bench_plain(args):
             t.main_loop()
             se = sum(t.retro_edges()[:2])
         out["cpu_baseline"] = cpu_reference_sample(stem, sk, se)
+        progress("reference sample done")
     if args.cpu_port < 0:
         args.cpu_port = 0 if args.no_cpu_baseline else min(16, len(os.sched_getaffinity(0)))
     mismatch = False
@@ -616,10 +625,13 @@ def bench_plain(args):
             out["check_vs_port"] = same
             mismatch = not same
         del states
+        progress("cpu port done")
         out["reference_mt"] = reference_mt(stem, 3, args.cpu_port)
+        progress("reference multithreaded solver done")
     s.close()
     if args.workload == "planar200_k4" and not args.no_batch:
         out["batch"] = run_batch(5, 2, 128, 0.7)[0]
+        progress("batch block done")
     print(json.dumps(out))
     if mismatch:
         print("bench: the device table differs from the CPU port's", file=sys.stderr)
