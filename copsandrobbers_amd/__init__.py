@@ -45,7 +45,7 @@ EXPORTS = (
     "cr_batch_create", "cr_batch_destroy", "cr_batch_solve", "cr_batch_set_tuning", "cr_batch_pass_counts",
     "cr_batch_last_timing", "cr_server_set_tuning",
     "cr_server_create", "cr_server_submit", "cr_server_submit_many", "cr_server_poll", "cr_server_stats", "cr_server_destroy",
-    "cr_clock_ns",
+    "cr_clock_ns", "cr_batch_solve_noalloc",
 )
 
 

@@ -52,6 +52,8 @@ def _lib():
         L.cr_server_destroy.argtypes = [vp]
         L.cr_server_destroy.restype = None
         L.cr_clock_ns.restype = u64
+        L.cr_batch_solve_noalloc.argtypes = [ctypes.c_int, i32, i32p, i32p, ctypes.POINTER(ctypes.c_uint8), u64p,
+                                             ctypes.POINTER(GraphResult), ctypes.c_void_p, u64, u64p]
         _ready = True
     return L
 
@@ -80,6 +82,33 @@ def _pack(graphs):
     if adj.size == 0:
         adj = np.zeros(1, np.uint8)
     return ns, ks, adj, offs
+
+
+NOALLOC_SCRATCH_BYTES = 256 << 20
+NOALLOC_MAX_GRAPHS = 4096
+
+
+def solve_noalloc(graphs, d_tables=None, tables_cap=0, device=0, raise_on_error=True):
+    """cr_batch_solve_noalloc: one launch over `graphs` with no allocation by
+    the library. d_tables is a device address (int) of tables_cap bytes owned
+    by the caller (e.g. a torch uint8 tensor's data_ptr()); graph i's table
+    lands at offsets[i]. With d_tables None only the offsets are computed
+    (offsets[-1] = bytes needed). Returns (results as dicts, offsets)."""
+    L = _lib()
+    ns, ks, adj, offs = _pack(graphs)
+    count = len(graphs)
+    res = (GraphResult * max(count, 1))()
+    toff = np.zeros(count + 1, np.uint64)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    rc = L.cr_batch_solve_noalloc(device, count, ns.ctypes.data_as(i32p), ks.ctypes.data_as(i32p), _u8(adj),
+                                  offs.ctypes.data_as(u64p), res, d_tables, tables_cap, toff.ctypes.data_as(u64p))
+    if rc != 0 and (raise_on_error or rc != -1):
+        _check(rc)
+    out = [to_dict(res[i]) for i in range(count)]
+    for i, r in enumerate(out):
+        r["positions"] = r["positions"][:int(ks[i])]
+    return out, toff
 
 
 class Batch:

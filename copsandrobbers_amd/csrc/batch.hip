@@ -26,6 +26,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstddef>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -511,8 +512,9 @@ struct Engine {
     }
 
     // Converts launched graph i's output (after the stream has completed).
-    void result(uint32_t i, const Item& it, cr_graph_result& r) const {
-        const BatchOut& o = h_out.as<BatchOut>()[i];
+    void result(uint32_t i, const Item& it, cr_graph_result& r) const { to_result(h_out.as<BatchOut>()[i], it, r); }
+
+    static void to_result(const BatchOut& o, const Item& it, cr_graph_result& r) {
         std::memset(&r, 0, sizeof r);
         r.status = CR_OK;
         r.base_captures = o.base;
@@ -913,3 +915,153 @@ void cr_server_destroy(cr_server* s) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// cr_batch_solve_noalloc: the batch launch with every work buffer in one
+// file-scope device array, for callers that cannot allocate.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t kNoAllocScratchBytes = CR_NOALLOC_SCRATCH_BYTES;
+constexpr int32_t kNoAllocMaxGraphs = CR_NOALLOC_MAX_GRAPHS;
+constexpr int kNoAllocMaxDevices = 64;
+
+// The device side of every call on a device: graph descriptors, work order,
+// robber masks, the work counter, per-graph outputs and the workgroups' work
+// planes. Each device holds its own copy until the process ends.
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+// Host staging in one block, uploaded by one copy (the counter is zeroed by
+// that copy). Static, so the call allocates nothing on the host either.
+struct NoAllocStage {
+    BatchGraph graphs[kNoAllocMaxGraphs];
+    uint32_t order[kNoAllocMaxGraphs];
+    uint32_t next[64];  // the work counter, alone in its 256 bytes
+    uint64_t masks[static_cast<size_t>(kNoAllocMaxGraphs) * kMaxBN];
+};
+static_assert(sizeof(BatchGraph) % 8 == 0 && offsetof(NoAllocStage, next) % 8 == 0 &&
+              offsetof(NoAllocStage, masks) % 8 == 0, "staging layout");
+
+NoAllocStage g_stage;
+BatchOut g_out[kNoAllocMaxGraphs];
+Item g_items[kNoAllocMaxGraphs];
+uint32_t g_valid[kNoAllocMaxGraphs];  // caller index of each launched graph
+
+// Layout of the device array (byte offsets; each 256-aligned).
+constexpr uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+constexpr uint64_t kOffStage = 0;
+constexpr uint64_t kOffOut = align256(kOffStage + sizeof(NoAllocStage));
+constexpr uint64_t kOffSlots = align256(kOffOut + sizeof(BatchOut) * kNoAllocMaxGraphs);
+static_assert(kOffSlots < kNoAllocScratchBytes, "scratch holds the staging, outputs and some work planes");
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on the current device `dev`, looked up once
+// per device. The array's host-side name is not a device address; only this
+// pointer is passed to kernels and copies.
+int scratch_on(int dev, uint8_t** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr)
+            return fail(CR_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = static_cast<uint8_t*>(g_scratch_ptr[dev]);
+    return CR_OK;
+}
+
+}  // namespace
+
+extern "C" int cr_batch_solve_noalloc(int device, int32_t count, const int32_t* n, const int32_t* k,
+                                      const uint8_t* adj, const uint64_t* adj_offset, cr_graph_result* results,
+                                      uint8_t* d_tables, uint64_t tables_cap, uint64_t* table_offsets) {
+    if (count < 0 || (count > 0 && (n == nullptr || k == nullptr || adj == nullptr || adj_offset == nullptr ||
+                                    results == nullptr)) || table_offsets == nullptr)
+        return fail(CR_ERR_ARG, "null argument");
+    if (count > kNoAllocMaxGraphs)
+        return fail(CR_ERR_TOO_LARGE, "at most %d graphs per no-allocation call (got %d)", kNoAllocMaxGraphs, count);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(CR_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= ndev || device >= kNoAllocMaxDevices) return fail(CR_ERR_ARG, "device %d of %d", device, ndev);
+    // validate and size every graph; tables are laid out 256-byte aligned in caller order
+    int first_bad = CR_OK;
+    char first_msg[512] = "";
+    uint32_t nvalid = 0;
+    uint64_t table_off = 0, mask_off = 0, slot_words = 64, max_c = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        Item& it = g_items[i];
+        it = Item{};
+        const int st = prepare(it, n[i], k[i], adj + adj_offset[i]);
+        std::memset(&results[i], 0, sizeof results[i]);
+        results[i].ticket = static_cast<uint64_t>(i);
+        results[i].status = st;
+        results[i].plys = -1;
+        table_offsets[i] = table_off;
+        if (st != CR_OK) {
+            if (first_bad == CR_OK) {
+                first_bad = st;
+                std::snprintf(first_msg, sizeof first_msg, "graph %d: %s", i, crd::g_err);
+            }
+            continue;
+        }
+        g_stage.graphs[nvalid] = BatchGraph{it.n, it.k, it.C, it.hmax, mask_off, table_off};
+        std::memcpy(g_stage.masks + mask_off, it.masks, it.n * 8ull);
+        g_valid[nvalid++] = static_cast<uint32_t>(i);
+        table_off += align256(it.table_bytes);
+        mask_off += it.n;
+        slot_words = std::max<uint64_t>(slot_words, (it.slot_words + 31) & ~31ull);
+        max_c = std::max<uint64_t>(max_c, it.C);
+    }
+    table_offsets[count] = table_off;
+    if (d_tables == nullptr) return first_bad == CR_OK ? CR_OK : fail(CR_ERR_ARG, "%s", first_msg);  // sizing only
+    if (tables_cap < table_off)
+        return fail(CR_ERR_TOO_LARGE, "the tables need %llu bytes; d_tables holds %llu",
+                    static_cast<unsigned long long>(table_off), static_cast<unsigned long long>(tables_cap));
+    const uint64_t slot_bytes = slot_words * 8;
+    const uint64_t slot_room = kNoAllocScratchBytes - kOffSlots;
+    if (nvalid > 0 && slot_bytes > slot_room)
+        return fail(CR_ERR_TOO_LARGE, "a graph needs %llu bytes of work planes; the no-allocation scratch has %llu",
+                    static_cast<unsigned long long>(slot_bytes), static_cast<unsigned long long>(slot_room));
+    // largest first, ties by position (std::sort: no temporary buffer)
+    for (uint32_t i = 0; i < nvalid; ++i) g_stage.order[i] = i;
+    std::sort(g_stage.order, g_stage.order + nvalid, [](uint32_t a, uint32_t b) {
+        const double ca = g_items[g_valid[a]].cost, cb = g_items[g_valid[b]].cost;
+        return ca != cb ? ca > cb : a < b;
+    });
+    std::memset(g_stage.next, 0, sizeof g_stage.next);
+    HIP_TRY(hipSetDevice(device));
+    uint8_t* base = nullptr;
+    if (int rc = scratch_on(device, &base)) return rc;
+    if (nvalid > 0) {
+        int cus = 1;
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        const uint32_t block = max_c >= 4096 ? 1024u : 256u;
+        uint64_t grid = std::min<uint64_t>(nvalid, static_cast<uint64_t>(std::max(cus, 1)) * (2048u / block));
+        grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, slot_room / slot_bytes));
+        // the staging block up to the masks in use, in one copy (graphs, order, zeroed counter, masks)
+        const uint64_t stage_bytes = offsetof(NoAllocStage, masks) + mask_off * 8;
+        HIP_TRY(hipMemcpyAsync(base + kOffStage, &g_stage, stage_bytes, hipMemcpyHostToDevice, nullptr));
+        NoAllocStage* ds = reinterpret_cast<NoAllocStage*>(base + kOffStage);
+        BatchOut* dout = reinterpret_cast<BatchOut*>(base + kOffOut);
+        uint64_t* slots = reinterpret_cast<uint64_t*>(base + kOffSlots);
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(grid)), dim3(block), 0, nullptr, ds->graphs, ds->order,
+                               nvalid, ds->masks, d_tables, dout, slots, slot_words, ds->next);
+        };
+        if (block == 1024) go(k_batch_solve<1024>);
+        else go(k_batch_solve<256>);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(g_out, dout, nvalid * sizeof(BatchOut), hipMemcpyDeviceToHost, nullptr));
+        HIP_TRY(hipStreamSynchronize(nullptr));
+    }
+    for (uint32_t i = 0; i < nvalid; ++i) {
+        const uint32_t ci = g_valid[i];
+        Engine::to_result(g_out[i], g_items[ci], results[ci]);
+        results[ci].ticket = ci;
+    }
+    if (first_bad != CR_OK) return fail(CR_ERR_ARG, "%s", first_msg);
+    return CR_OK;
+}

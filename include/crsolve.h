@@ -327,6 +327,28 @@ int cr_batch_pass_counts(cr_batch* b, int32_t index, uint64_t* out, int32_t cap,
 /* Device time of the last cr_batch_solve (launch to last result) and its grid. */
 int cr_batch_last_timing(cr_batch* b, double* device_ms, int32_t* workgroups);
 
+/* No counterpart in the reference: a variant of cr_batch_solve for callers
+ * that cannot allocate memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc. It makes no device or host
+ * allocation: the graph descriptors, robber masks, work counter, per-graph
+ * outputs and the workgroups' work planes live in one file-scope device array
+ * of CR_NOALLOC_SCRATCH_BYTES, which each device it runs on holds until the
+ * process ends. Inputs as in cr_batch_solve (host arrays, at most
+ * CR_NOALLOC_MAX_GRAPHS graphs). Graph i's state table is written to the
+ * caller's device memory d_tables + table_offsets[i]; the offsets are
+ * computed here (each table 256-byte aligned, table_offsets[count] = the bytes
+ * needed; table_offsets has count + 1 entries). With d_tables NULL the call
+ * only computes the offsets. CR_ERR_TOO_LARGE when count, tables_cap or one
+ * graph's work planes exceed these limits. Runs on the device's null stream
+ * and returns when the results are in `results`. Calls that overlap on one
+ * device (and any concurrent calls: the host staging is shared) race; the
+ * array keeps the last call's contents. */
+#define CR_NOALLOC_SCRATCH_BYTES (256ull << 20)
+#define CR_NOALLOC_MAX_GRAPHS 4096
+int cr_batch_solve_noalloc(int device, int32_t count, const int32_t* n, const int32_t* k, const uint8_t* adj,
+                           const uint64_t* adj_offset, cr_graph_result* results, uint8_t* d_tables,
+                           uint64_t tables_cap, uint64_t* table_offsets);
+
 /* Continuous batching: a dispatcher thread admits pending requests into a new
  * launch whenever one of `streams` launch slots is free and their footprint
  * (state table + per-graph work planes, bytes) fits in what `hbm_budget`
