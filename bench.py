@@ -74,7 +74,7 @@ MACRO_WORKLOADS = {
     "grid8x8_k2_p2": ("grid8x8_chords16_s1", 2, 2),
 }
 # solver timer name -> device kernel name in rocprofv3 summaries
-KERNEL_NAMES = {"cop_push": "k_cop_push", "sep_stage": "k_pull_tile + k_pull_tile1", "robber_step": "k_robber_step",
+KERNEL_NAMES = {"cop_push": "k_cop_push", "sep_stage": "k_pull_tile + k_pull_tile1 + k_pull_pair", "robber_step": "k_robber_step",
                 "cop_finalize": "k_cop_finalize", "clear_frontier": "k_clear_frontier",
                 "mx_expand": "k_mx_expand", "mx_emit": "k_mx_expand", "mx_insert": "k_mx_insert",
                 "mx_compact": "k_mx_compact", "mx_leaf": "k_mx_leaf", "mx_up": "k_mx_up",
@@ -481,6 +481,8 @@ def main():
                     help="threads for the oracle-port timing (default: min(16, cores); 0 = skip)")
     ap.add_argument("--cpu-port-cap", type=float, default=20.0,
                     help="seconds the CPU port may run on workloads above 2e9 states")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="plain workloads: a solver speed setting (copsandrobbers_amd.TUNING names)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (rocprof counter calibration)")
     args = ap.parse_args()
@@ -506,6 +508,8 @@ def bench_plain(args):
     stem, k = WORKLOADS[args.workload]
     path = matrix_path(stem)
     s = cra.Solver(0)
+    tuning = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.tune}
+    s.set_tuning(**tuning)
     s.load_graph_file(path, k, 1)
     s.build_aux_graph()
     info = s.info()
@@ -585,7 +589,7 @@ def bench_plain(args):
                    "e_cop": e_cop, "e_rob": e_rob, "won_states": won, "levels": stats.n_levels,
                    "passes": len(passes), "verdict": "WIN" if verdict else "LOSS",
                    "positions": list(verdict["positions"]) if verdict else None,
-                   "plys": verdict["plys"] if verdict else None},
+                   "plys": verdict["plys"] if verdict else None, "tuning": tuning},
         "performed_gteps": {"value": performed * args.steps / elapsed / 1e9, "relaxations": performed,
                             "note": "relaxations from states of depth >= 1; those of the depth-0 captures are "
                                     "replaced by the closed-form initialisation"},

@@ -467,6 +467,10 @@ struct SepStage {
     uint32_t chunk;       // k_pull_tile: tiles a workgroup takes at a time
     int32_t fuse;         // stage K: do k_cop_finalize's work instead of writing nx0
     int32_t clear;        // stage 2: clear the frontier rows of fr1 (k_clear_frontier's work)
+    // k_pull_pair (stages K-1 and K in one pass): tiles (A-range R, c), R-major
+    const uint32_t* tstart;  // [nR + 1]: first tile of A-range R
+    const uint32_t* cstart;  // [nR]: least max A over R (the first c of R's tiles)
+    uint32_t nR;             // A-ranges of r consecutive A
 };
 
 // The stage's input rows: stage 1 reads the robber frontier of the level's depth.
@@ -1085,6 +1089,181 @@ __global__ __launch_bounds__(kTile1Block) void k_pull_tile1(GraphView gv, LevelA
         if (!more) break;
         __syncthreads();  // every thread is done with the current tile: the next one replaces it
         store_tile();
+    }
+}
+
+// Stages K-1 and K of the pull in one pass (K >= 3). With I2 = K - 2 cops in
+// A and the pair B2 = {b, m}:
+//   H_{K-1}[A + c][{b}] = OR_{m in N(c)} H_{K-2}[A][{b, m}]      (c >= max A)
+//   H_K[A + c + a]      = OR_{b in N(a)} H_{K-1}[A + c][{b}]       (a >= c)
+// For a fixed A and c the first line is only n rows, so a tile (A-range R of
+// r consecutive A, one c) builds them in LDS (Y[t][b], phase 1) and ORs the
+// outputs out of LDS (phase 2): H_{K-1} is never written to memory. The
+// outputs of a tile are, per A_t, the consecutive configurations
+// A_t + c + c .. A_t + c + (n - 1). Phase 1 reads, for each m in N(c) and
+// every b, the rows of R in block {b, m} of H_{K-2} (B major: r consecutive
+// rows). Tiles are numbered R-major, c-minor, and split into 8 contiguous
+// ranges, one per XCD (workgroup w runs on XCD w mod 8): the workgroups of
+// an XCD work on neighbouring tiles, which share R, so the rows of R's blocks
+// (n (n + 1) / 2 * r rows) are re-read from that XCD's L2.
+constexpr int kPairBlock = 256;
+constexpr int kPairMaxRows = 8;
+constexpr int kPairItems = 8;   // phase-1 items per thread: r * n * W / (2 or 1) <= kPairItems * kPairBlock
+constexpr int kPairUnroll = 2;  // neighbours whose loads a phase-1 step issues together
+
+template <int I2, int W>
+__global__ __launch_bounds__(kPairBlock) void k_pull_pair(GraphView gv, LevelArgs a, SepStage st) {
+    static_assert(I2 >= 1, "k_pull_pair takes K >= 3");
+    constexpr int K = I2 + 2;
+    constexpr int WH = (W % 2 == 0) ? 2 : 1;  // words per phase-1 item (16-byte loads where rows allow)
+    constexpr int NI = W / WH;                // items per (t, b)
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ unsigned long long s_base[kPairMaxRows];  // configuration of A_t + c + c, ~0 if max A_t > c
+    if (a.ctl->done || !a.ctl->pull) return;  // uniform: before any barrier
+    const uint64_t* sin = st.in;
+    load_image(lds, gv.image, gv.image_bytes);
+    const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
+    const uint16_t* out_off = reinterpret_cast<const uint16_t*>(lds + gv.off_out_off);
+    const uint8_t* out_nbr = lds + gv.off_out_nbr;
+    const uint32_t n = gv.n;
+    const uint32_t r = st.r;
+    uint64_t* Y = reinterpret_cast<uint64_t*>(lds + ((gv.image_bytes + 15) & ~15u));
+    uint64_t full[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        const int bits = min(64, max(0, static_cast<int>(n) - 64 * q));
+        full[q] = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+    }
+    if (st.clear) {  // K = 3: this is stage 2, after stage 1 has read the frontier (k_clear_frontier's work)
+        const uint32_t cur = a.ctl->level & 1;
+        const uint64_t cw = static_cast<uint64_t>(a.ctl->cnt1[cur]) * W;
+        const uint32_t* list = a.list1[cur];
+        for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < cw;
+             i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+            a.fr1s[cur][static_cast<uint64_t>(list[i / W]) * W + i % W] = 0;
+    }
+    const uint32_t cnt_a = st.cnt_a_in;  // A-multisets of H_{K-2}: rows per block
+    const uint32_t T = st.tstart[st.nR];
+    const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, l = blockIdx.x / kXcds;
+    const uint32_t t_lo = static_cast<uint32_t>(static_cast<uint64_t>(T) * xcd / kXcds);
+    const uint32_t t_hi = static_cast<uint32_t>(static_cast<uint64_t>(T) * (xcd + 1) / kXcds);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t R = 0;
+    for (uint32_t tile = t_lo + l; tile < t_hi; tile += per_xcd) {
+        // the tile's A-range: the last R with tstart[R] <= tile (tiles only move forward)
+        for (uint32_t hi = st.nR - 1; R < hi;) {
+            const uint32_t mid = (R + hi + 1) >> 1;
+            if (st.tstart[mid] <= tile) R = mid; else hi = mid - 1;
+        }
+        const uint32_t c = st.cstart[R] + (tile - st.tstart[R]);
+        const uint32_t ra0 = R * r, rc = min(r, cnt_a - ra0);
+        if (threadIdx.x < rc) {
+            uint32_t A[K];
+            unrank_j(ra0 + threadIdx.x, I2, n, binom, gv.M, A);
+            unsigned long long base = ~0ull;
+            if (A[I2 - 1] <= c) {
+                A[I2] = c;
+                A[I2 + 1] = c;
+                base = rank_j(A, K, n, binom, gv.M);
+            }
+            s_base[threadIdx.x] = base;
+        }
+        __syncthreads();
+        // phase 1: Y[t][b] = OR_{m in N(c)} H_{K-2}[A_t][{b, m}]. Each thread
+        // keeps up to kPairItems (t, b, word pair) items and, per neighbour m,
+        // issues the loads of all of them before ORing (kPairItems loads in flight).
+        const int beg = out_off[c], end = out_off[c + 1];
+        const uint32_t items = rc * n * NI;
+        uint64_t acc[kPairItems][WH];
+        uint32_t ib[kPairItems];   // b of the item, n if the thread has no such item
+        uint64_t ioff[kPairItems]; // word offset of the item's row within a block of H_{K-2}
+#pragma unroll
+        for (int it = 0; it < kPairItems; ++it) {
+            const uint32_t i = threadIdx.x + it * kPairBlock;
+            const uint32_t q = i % NI, t = (i / NI) % rc, b = i / (NI * rc);
+            const bool ok = i < items && s_base[t] != ~0ull;
+            ib[it] = ok ? b : n;
+            ioff[it] = static_cast<uint64_t>(ra0 + t) * W + q * WH;
+#pragma unroll
+            for (int h = 0; h < WH; ++h) acc[it][h] = 0;
+        }
+        // kPairUnroll neighbours per step (a step past the last neighbour repeats
+        // it: ORing a row twice changes nothing), so that their loads are in
+        // flight together
+        for (int e0 = beg; e0 < end; e0 += kPairUnroll) {
+#pragma unroll
+            for (int u = 0; u < kPairUnroll; ++u) {
+                const uint32_t m = out_nbr[min(e0 + u, end - 1)];
+#pragma unroll
+                for (int it = 0; it < kPairItems; ++it) {
+                    const uint32_t b = ib[it];
+                    if (b < n) {
+                        const uint32_t x = min(b, m), y = max(b, m);
+                        const uint32_t pr = x * n - (x * (x - 1)) / 2 + (y - x);  // rank of the pair {x <= y}
+                        const uint64_t* src = sin + static_cast<uint64_t>(pr) * cnt_a * W + ioff[it];
+                        if constexpr (WH == 2) {
+                            const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(src);
+                            acc[it][0] |= v.x;
+                            acc[it][1] |= v.y;
+                        } else {
+                            acc[it][0] |= src[0];
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < kPairItems; ++it) {
+            if (ib[it] < n) {
+                const uint32_t i = threadIdx.x + it * kPairBlock;
+                const uint32_t q = i % NI, t = (i / NI) % rc;
+                uint64_t* dst = Y + (static_cast<uint64_t>(t) * n + ib[it]) * W + q * WH;
+                if constexpr (WH == 2) {
+                    *reinterpret_cast<ulonglong2*>(dst) = make_ulonglong2(acc[it][0], acc[it][1]);
+                } else {
+                    dst[0] = acc[it][0];
+                }
+            }
+        }
+        __syncthreads();
+        // phase 2: one wave per A_t, lanes over consecutive outputs a = c + j
+        for (uint32_t t = wave; t < rc; t += kPairBlock / 64) {
+            const unsigned long long o0 = s_base[t];
+            if (o0 == ~0ull) continue;  // uniform per wave
+            const uint64_t* yt = Y + static_cast<uint64_t>(t) * n * W;
+            for (uint32_t j0 = 0; j0 < n - c; j0 += 64) {
+                const uint32_t av = c + j0 + lane;
+                const bool valid = av < n;
+                uint64_t acc[W];
+#pragma unroll
+                for (int q = 0; q < W; ++q) acc[q] = 0;
+                uint32_t pop = 0;
+                const uint64_t o = o0 + j0 + lane;
+                if (valid) {
+                    uint64_t w0[W];
+                    load_row<W>(a.won0 + o * W, w0);  // in flight during the ORs
+                    const int b0 = out_off[av], b1 = out_off[av + 1];
+                    for (int e = b0; e < b1; ++e) {
+                        uint64_t v[W];
+                        load_row<W>(yt + static_cast<uint32_t>(out_nbr[e]) * W, v);
+#pragma unroll
+                        for (int q = 0; q < W; ++q) acc[q] |= v[q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < W; ++q) {
+                        acc[q] &= ~w0[q] & full[q];
+                        pop += __popcll(acc[q]);
+                    }
+                }
+                if (st.fuse) {  // as the tiled last stage: new states to fr0, flags and group counts
+                    if (pop) store_row<W>(st.out + o * W, acc);
+                    fuse_flags(a, o0 + j0, __ballot(pop != 0), pop);
+                } else if (valid) {
+                    store_row<W>(st.out + o * W, acc);
+                }
+            }
+        }
+        __syncthreads();  // Y and s_base are rewritten by the next tile
     }
 }
 
@@ -2120,6 +2299,13 @@ struct cr_solver {
     bool push_dirty = true;
     unsigned long long* d_tile_states = nullptr;
     uint32_t pull_gate = 4;                   // a chunk may pull if its first frontier * pull_gate > pull_rows
+    int32_t pull_pair = 1;                    // tuning: the last two pull stages as one kernel (k_pull_pair): 0 never, 1 k = 3, 2 k >= 3
+    uint32_t pair_rows = 4;                   // tuning: A-multisets per k_pull_pair tile
+    uint32_t pair_blocks = 1024;              // tuning: k_pull_pair workgroups
+    bool pair_ready = false;                  // k_pull_pair runs this main loop (tile tables below built)
+    uint32_t pair_r = 0, pair_nR = 0;         // tile rows and A-ranges of the built tables
+    uint32_t* d_pair = nullptr;               // [pair_nR + 1] first tile of each A-range, then [pair_nR] its first c
+    uint64_t pair_cap = 0;                    // words of d_pair
     bool fuse_finalize = true;                // tuning: with the depth log, pulled levels finalize in the last stage
     bool fuse_ok = false;                     // the last pull stage runs as LDS tiles (image and one row per vertex fit)
     bool stage2_cleared = false;              // the pull being enqueued clears the frontier in stage 2
@@ -2182,6 +2368,11 @@ struct cr_solver {
         hipFree(d_tile_cnt);
         hipFree(d_dirty);
         d_dirty = nullptr;
+        if (d_pair) hipFree(d_pair);
+        d_pair = nullptr;
+        pair_cap = 0;
+        pair_ready = false;
+        pair_r = 0;
         hipFree(d_tile_states);
         d_tile_cnt = nullptr;
         d_tile_states = nullptr;
@@ -2380,6 +2571,40 @@ auto pull_tile1_kernel() {
         return k_pull_tile<1, 0, W>;
 }
 
+// Stages K-1 and K of one pull as one k_pull_pair launch (prepare_pair built
+// its tile tables for this loop); `in` is H_{K-2}.
+template <int K, int W>
+int launch_pull_pair(cr_solver* s, LevelArgs a, const uint64_t* in) {
+    if constexpr (K < 3) {
+        return fail(CR_ERR_DEVICE, "k_pull_pair needs k >= 3");
+    } else {
+        SepStage st{};
+        st.r = s->pair_r;
+        st.fuse = a.fuse;
+        st.clear = K == 3;  // this launch is stage 2, after stage 1 has read the frontier
+        if (st.clear) s->stage2_cleared = true;
+        if (st.fuse) hipLaunchKernelGGL(k_zero_groups, dim3(1024), dim3(kBlock), 0, s->stream, a);
+        st.stage = K;
+        st.in = in;
+        st.out = st.fuse ? s->d_fr0 : s->d_nx0;
+        st.cnt_a_in = static_cast<uint32_t>(s->ms_count[K - 2]);
+        st.cnt_a = static_cast<uint32_t>(s->ms_count[K]);
+        st.rows = s->ms_count[K];
+        st.nR = s->pair_nR;
+        st.tstart = s->d_pair;
+        st.cstart = s->d_pair + s->pair_nR + 1;
+        auto kern = k_pull_pair<K - 2, W>;
+        const void* fn = reinterpret_cast<const void*>(kern);
+        const uint32_t lds = ((s->gv.image_bytes + 15) & ~15u) + s->g.n * st.r * W * 8;
+        if (int rc = raise_lds_limit(fn, lds)) return rc;
+        const uint32_t blocks = std::max<uint32_t>(kXcds, s->pair_blocks / kXcds * kXcds);
+        Timer* t = start_timer(s, "sep_stage");
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kPairBlock), lds, s->stream, s->gv, a, st);
+        t->stop();
+        return 0;
+    }
+}
+
 // Stages I..K of one pull; stage I reads `in` (H_{I-1}) and writes H_I.
 // The grid is a multiple of the XCD count (k_sep_stage splits its windows
 // per XCD).
@@ -2388,6 +2613,9 @@ int launch_pull_stage(cr_solver* s, LevelArgs a, const uint64_t* in) {
     if constexpr (I > K) {
         return 0;
     } else {
+        if constexpr (I == K - 1 && K >= 3) {
+            if (s->pair_ready) return launch_pull_pair<K, W>(s, a, in);
+        }
         SepStage st{};
         // k_pull_tile1 takes stage 1 of a multi-cop pull (r consecutive B per tile)
         const bool tile1 = I == 1 && K > 1 && s->tile1_rows > 0;
@@ -2684,6 +2912,67 @@ int alloc_pull(cr_solver* s) {
     if (!(s->d_sep[0] = dev_alloc<uint64_t>(std::max<uint64_t>(need[0] / 8, 1), s->device_bytes, rc))) return rc;
     if (!(s->d_sep[1] = dev_alloc<uint64_t>(std::max<uint64_t>(need[1] / 8, 1), s->device_bytes, rc))) return rc;
     set_pull_threshold(s);
+    return 0;
+}
+
+// Tile tables of k_pull_pair for the current graph and pair_rows: A-ranges
+// R of r consecutive (K-2)-multisets, each with tiles c = (least max A over
+// R) .. n - 1, numbered R-major. Built on the host before a main loop; the
+// kernel is used where the tiled stages are and its LDS tile fits.
+int prepare_pair(cr_solver* s) {
+    const int n = s->g.n, K = s->k;
+    const uint32_t img = (s->gv.image_bytes + 15) & ~15u;
+    uint32_t r = std::min<uint32_t>(std::max<uint32_t>(s->pair_rows, 1), kPairMaxRows);
+    const uint32_t ni = s->W % 2 == 0 ? s->W / 2 : s->W;  // phase-1 items per (A, b)
+    while (r > 1 && (img + static_cast<uint64_t>(n) * r * s->W * 8 > kLdsBytes ||
+                     static_cast<uint64_t>(n) * r * ni > static_cast<uint64_t>(kPairItems) * kPairBlock))
+        --r;
+    const bool usable = (s->pull_pair == 2 || (s->pull_pair == 1 && K == 3)) && s->pull_tiles && s->d_sep[0] != nullptr && K >= 3 &&
+                        img + static_cast<uint64_t>(n) * r * s->W * 8 <= kLdsBytes &&
+                        static_cast<uint64_t>(n) * r * ni <= static_cast<uint64_t>(kPairItems) * kPairBlock &&
+                        s->ms_count[K - 2] <= (1u << 24);
+    s->pair_ready = false;
+    if (!usable) return 0;
+    if (s->pair_r == r) {
+        s->pair_ready = true;
+        return 0;
+    }
+    const uint32_t cnt = static_cast<uint32_t>(s->ms_count[K - 2]);
+    const uint32_t nR = (cnt + r - 1) / r;
+    std::vector<uint32_t> h(2ull * nR + 1);
+    std::vector<uint32_t> A(K - 2, 0u);
+    uint64_t total = 0;
+    for (uint32_t R = 0; R < nR; ++R) {
+        uint32_t cmin = static_cast<uint32_t>(n);
+        for (uint32_t t = 0; t < r && R * r + t < cnt; ++t) {
+            cmin = std::min<uint32_t>(cmin, A[K - 3]);
+            // lexicographic successor of the non-decreasing tuple A
+            int i = K - 3;
+            while (i >= 0 && A[i] + 1 >= static_cast<uint32_t>(n)) --i;
+            if (i >= 0) {
+                const uint32_t v = A[i] + 1;
+                for (int j = i; j < K - 2; ++j) A[j] = v;
+            }
+        }
+        h[R] = static_cast<uint32_t>(total);
+        h[nR + 1 + R] = cmin;
+        total += static_cast<uint32_t>(n) - cmin;
+    }
+    if (total >= 0xffffffffull) return 0;  // stage by stage instead
+    h[nR] = static_cast<uint32_t>(total);
+    if (s->pair_cap < h.size()) {
+        if (s->d_pair) hipFree(s->d_pair);
+        s->d_pair = nullptr;
+        s->pair_cap = 0;
+        int rc = 0;
+        if (!(s->d_pair = dev_alloc<uint32_t>(h.size(), s->device_bytes, rc))) return rc;
+        s->pair_cap = h.size();
+    }
+    HIP_TRY(hipMemcpyAsync(s->d_pair, h.data(), h.size() * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->pair_r = r;
+    s->pair_nR = nR;
+    s->pair_ready = true;
     return 0;
 }
 
@@ -3089,6 +3378,7 @@ int do_loop(cr_solver* s) {
     s->korder.clear();
     if (s->variant == CR_VARIANT_MACRO) return loop_macro(s);
     if (s->p > 1) return loop_columns(s);
+    if (int rc = prepare_pair(s)) return rc;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -3690,6 +3980,18 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             return 0;
         case CR_TUNE_FUSE_FINALIZE: s->fuse_finalize = value != 0; return 0;
         case CR_TUNE_PUSH_DIRTY: s->push_dirty = value != 0; return 0;
+        case CR_TUNE_PULL_PAIR:
+            if (value < 0 || value > 2) return fail(CR_ERR_ARG, "pull pair %lld", static_cast<long long>(value));
+            s->pull_pair = static_cast<int32_t>(value);
+            return 0;
+        case CR_TUNE_PAIR_ROWS:
+            if (value < 1 || value > kPairMaxRows) return fail(CR_ERR_ARG, "pair tile rows %lld", static_cast<long long>(value));
+            s->pair_rows = static_cast<uint32_t>(value);
+            return 0;
+        case CR_TUNE_PAIR_BLOCKS:
+            if (value < 1 || value > (1 << 20)) return fail(CR_ERR_ARG, "pair blocks %lld", static_cast<long long>(value));
+            s->pair_blocks = static_cast<uint32_t>(value);
+            return 0;
         case CR_TUNE_SCAN_TILE:
             if (value < kScanTileMin || value > kScanTileMax) return fail(CR_ERR_ARG, "scan tile %lld", static_cast<long long>(value));
             s->scan_tile = static_cast<uint32_t>(value);

@@ -258,8 +258,13 @@ enum cr_tuning {
                                     depth log (default 4096) */
     CR_TUNE_SCAN_TILE = 18,      /* list groups per tile of the level scan, 4..8192 (default 8192): with
                                     more groups than one tile, tiles are scanned in parallel first */
-    CR_TUNE_PUSH_DIRTY = 19      /* 1 (default): the push marks the groups of 256 configurations it
+    CR_TUNE_PUSH_DIRTY = 19,     /* 1 (default): the push marks the groups of 256 configurations it
                                     writes, and the finalize of a push level reads only those */
+    CR_TUNE_PULL_PAIR = 20,      /* a tiled pull runs its last two stages as one kernel that keeps the
+                                    next-to-last table in LDS: 0 never, 1 (default) for k = 3 (measured
+                                    faster there; no gain at k = 4), 2 for every k >= 3 */
+    CR_TUNE_PAIR_ROWS = 21,      /* cop multisets per tile of that kernel, 1..8 (default 4) */
+    CR_TUNE_PAIR_BLOCKS = 22     /* workgroups of that kernel, rounded to a multiple of 8 (default 1024) */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:

@@ -27,6 +27,9 @@ CHOICES = {
     "pred_csr": [0, 0, 1],
     "push_filter": [0, 1, 1],
     "push_blocks": [8, 2048],
+    "pull_pair": [0, 1, 2, 2],
+    "pair_rows": [1, 3, 4, 8],
+    "pair_blocks": [8, 1024],
 }
 
 

@@ -49,8 +49,9 @@ def er_k3(oracle):
     return n, adj, states, levels, e_cop, e_rob
 
 
-ER_SETTINGS = [{}, {"depth_log_min": 0, "scan_tile": 4}, {"pull_scale": 0}, {"pull_scale": 10**9, "pred_csr": 1}]
-ER_IDS = ["default", "depth_log_two_level_scan", "pull_every_level", "push_csr_every_level"]
+ER_SETTINGS = [{}, {"depth_log_min": 0, "scan_tile": 4, "pull_pair": 0}, {"pull_scale": 0}, {"pull_scale": 0, "pull_pair": 0},
+               {"pull_scale": 10**9, "pred_csr": 1}]
+ER_IDS = ["default", "depth_log_two_level_scan", "pull_every_level", "pull_stage_by_stage", "push_csr_every_level"]
 
 
 @pytest.mark.parametrize("setting", ER_SETTINGS, ids=ER_IDS)
