@@ -304,7 +304,7 @@ def bench_macro(args):
     return 0
 
 
-def run_batch(steps, warmup, plain_sample, serve_load):
+def run_batch(steps, warmup, plain_sample, serve_load, autotune_now=False, db=None):
     """4096 seeded connected graphs (n uniform in [8, 48], edge probability in
     [0.08, 0.3], k in {1, 2, 3}), full information. A step = one
     cr_batch_solve over all of them (one persistent launch; tables stay in
@@ -321,9 +321,22 @@ def run_batch(steps, warmup, plain_sample, serve_load):
     from copsandrobbers_amd import batch as crb
     from copsandrobbers_amd import workloads
 
+    from copsandrobbers_amd import autotune
     graphs = workloads.batch_graphs(4096)
     states = sum(2 * __import__("math").comb(a.shape[0] + k - 1, k) * a.shape[0] for a, k in graphs)
     b = crb.Batch(0)
+    key = autotune.batch_key(graphs)
+    tuning = {"key": key, "source": "defaults", "settings": {}}
+    if autotune_now:
+        best, t_best, t_def, evals = autotune.tune_batch(b, graphs, log=progress)
+        autotune.save_entry(key, autotune.entry(best, t_best, t_def, evals, "MI355X"), db)
+        tuning.update(source="autotuned in this run", settings=best, tuned_ms=t_best * 1e3, default_ms=t_def * 1e3)
+    else:
+        stored = autotune.load_db(db).get(key)
+        if stored is not None:
+            tuning.update(source="tuning database", settings=stored["settings"], tuned_ms=stored.get("best_ms"),
+                          default_ms=stored.get("default_ms"))
+    b.set_tuning(**tuning["settings"])
     for _ in range(warmup):
         ref = b.solve(graphs)
     torch.cuda.synchronize()
@@ -360,6 +373,7 @@ def run_batch(steps, warmup, plain_sample, serve_load):
     order = rng.permutation(len(graphs))[:nreq]
     results = {}
     with crb.Server(0, hbm_budget=64 << 30, max_batch=1024, streams=4) as srv:
+        srv.set_tuning(**tuning["settings"])
         # warm-up burst (untimed): every launch slot grows its buffers to working size
         srv.submit_many(graphs)
         warm = 0
@@ -399,7 +413,7 @@ def run_batch(steps, warmup, plain_sample, serve_load):
     dev_s = dev_ms / 1e3 / steps
     block = {
         "graphs_per_s": batched, "ms_per_step": elapsed * 1e3 / steps, "device_ms_per_step": dev_ms / steps,
-        "graphs": len(graphs), "states": states, "workgroups": grid,
+        "graphs": len(graphs), "states": states, "workgroups": grid, "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": table_bytes / dev_s / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": table_bytes / dev_s / HBM_PEAK,
                      "traffic": measured_traffic("batch4096", "k_batch_solve"), "kernel": "k_batch_solve",
@@ -418,14 +432,16 @@ def run_batch(steps, warmup, plain_sample, serve_load):
 
 
 def bench_batch(args):
-    block, graphs, ref, idx = run_batch(args.steps, args.warmup, args.plain_sample, args.serve_load)
+    block, graphs, ref, idx = run_batch(args.steps, args.warmup, args.plain_sample, args.serve_load, args.autotune,
+                                        args.tuning_db)
     out = {
         "metric": "batched graphs/s (4096 small graphs, one launch)", "value": block["graphs_per_s"],
         "unit": "graphs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": block["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u8/u64 bitsets (integer)", "data": "synthetic (seeded graph generator)",
         "config": {"workload": "batch4096", "graphs": block["graphs"], "states": block["states"],
-                   "workgroups": block["workgroups"], "device_ms_per_step": block["device_ms_per_step"]},
+                   "workgroups": block["workgroups"], "device_ms_per_step": block["device_ms_per_step"],
+                   "tuning": block["tuning"]},
         "roofline": block["roofline"], "plain_loop": block["plain_loop"], "serving": block["serving"],
         "host": host_info(), "cpu_baseline": None,
     }
@@ -481,6 +497,10 @@ def main():
                     help="threads for the oracle-port timing (default: min(16, cores); 0 = skip)")
     ap.add_argument("--cpu-port-cap", type=float, default=20.0,
                     help="seconds the CPU port may run on workloads above 2e9 states")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner for this workload (and the batch block) before timing and store its "
+                         "choice in the tuning database")
+    ap.add_argument("--tuning-db", default=None, help="tuning database (default: the package's)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="plain workloads: a solver speed setting (copsandrobbers_amd.TUNING names)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -507,12 +527,29 @@ def bench_plain(args):
 
     stem, k = WORKLOADS[args.workload]
     path = matrix_path(stem)
+    from copsandrobbers_amd import autotune
     s = cra.Solver(0)
-    tuning = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.tune}
-    s.set_tuning(**tuning)
     s.load_graph_file(path, k, 1)
     s.build_aux_graph()
     info = s.info()
+    with open(path, "rb") as f:
+        edges = int(cra.parse_matrix(f.read())[1].sum())
+    key = autotune.plain_entry_key(s, edges)
+    tuning = {"key": key, "source": "defaults", "settings": {}}
+    if args.autotune:
+        best, t_best, t_def, evals = autotune.tune_plain(s, log=progress)
+        autotune.save_entry(key, autotune.entry(best, t_best, t_def, evals, "MI355X"), args.tuning_db)
+        tuning.update(source="autotuned in this run", settings=best, tuned_ms=t_best * 1e3,
+                      default_ms=t_def * 1e3, evaluations=len(evals))
+    else:
+        stored = autotune.load_db(args.tuning_db).get(key)
+        if stored is not None:
+            tuning.update(source="tuning database", settings=stored["settings"],
+                          tuned_ms=stored.get("best_ms"), default_ms=stored.get("default_ms"))
+    overrides = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.tune}
+    if overrides:
+        tuning["overrides"] = overrides
+    s.set_tuning(**{**tuning["settings"], **overrides})
     progress(f"{args.workload}: {info.num_states} states, device buffers {info.device_bytes} B")
     for _ in range(args.warmup):
         s.initialize_captures()
@@ -634,7 +671,7 @@ def bench_plain(args):
         progress("reference multithreaded solver done")
     s.close()
     if args.workload == "planar200_k4" and not args.no_batch:
-        out["batch"] = run_batch(5, 2, 128, 0.7)[0]
+        out["batch"] = run_batch(5, 2, 128, 0.7, args.autotune, args.tuning_db)[0]
         progress("batch block done")
     print(json.dumps(out))
     if mismatch:

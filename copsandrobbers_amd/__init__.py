@@ -46,7 +46,8 @@ EXPORTS = (
     "cr_batch_create", "cr_batch_destroy", "cr_batch_solve", "cr_batch_set_tuning", "cr_batch_pass_counts",
     "cr_batch_last_timing", "cr_server_set_tuning",
     "cr_server_create", "cr_server_submit", "cr_server_submit_many", "cr_server_poll", "cr_server_stats", "cr_server_destroy",
-    "cr_clock_ns", "cr_batch_solve_noalloc",
+    "cr_clock_ns", "cr_batch_solve_noalloc", "cr_server_retain_tables", "cr_server_copy_table",
+    "cr_server_release_table", "cr_server_table_stats",
 )
 
 

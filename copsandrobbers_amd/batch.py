@@ -52,6 +52,10 @@ def _lib():
         L.cr_server_destroy.argtypes = [vp]
         L.cr_server_destroy.restype = None
         L.cr_clock_ns.restype = u64
+        L.cr_server_retain_tables.argtypes = [vp, i32]
+        L.cr_server_copy_table.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_uint8), u64, u64p]
+        L.cr_server_release_table.argtypes = [vp, u64]
+        L.cr_server_table_stats.argtypes = [vp, u64p, u64p, u64p]
         L.cr_batch_solve_noalloc.argtypes = [ctypes.c_int, i32, i32p, i32p, ctypes.POINTER(ctypes.c_uint8), u64p,
                                              ctypes.POINTER(GraphResult), ctypes.c_void_p, u64, u64p]
         _ready = True
@@ -215,6 +219,26 @@ class Server:
     def set_tuning(self, **knobs):
         for name, value in knobs.items():
             _check(_lib().cr_server_set_tuning(self._h, TUNING[name], int(value)))
+
+    def retain_tables(self, on=True):
+        """Keep each finished request's state table resident (within the HBM budget)."""
+        _check(_lib().cr_server_retain_tables(self._h, 1 if on else 0))
+
+    def table(self, ticket):
+        """The resident table of a finished request (uint8, the reference's layout)."""
+        n = ctypes.c_uint64()
+        _check(_lib().cr_server_copy_table(self._h, int(ticket), None, 0, ctypes.byref(n)))
+        out = np.empty(max(int(n.value), 1), np.uint8)
+        _check(_lib().cr_server_copy_table(self._h, int(ticket), _u8(out), out.size, ctypes.byref(n)))
+        return out[:int(n.value)]
+
+    def release(self, ticket):
+        _check(_lib().cr_server_release_table(self._h, int(ticket)))
+
+    def table_stats(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(_lib().cr_server_table_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"resident_tables": int(a.value), "resident_bytes": int(b.value), "evicted_launches": int(c.value)}
 
     def submit(self, adj, k, arrival_ns=0):
         adj = np.ascontiguousarray(adj, np.uint8)

@@ -31,7 +31,9 @@
 #include <deque>
 #include <mutex>
 #include <numeric>
+#include <memory>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "cr_device.h"
@@ -357,6 +359,11 @@ int grow(Buf& b, uint64_t bytes, bool pinned) {
     return 0;
 }
 
+struct TableRefView {
+    const void* base;
+    uint64_t offset, bytes;
+};
+
 void drop(Buf& b) {
     if (b.p) {
         if (b.pinned) hipHostFree(b.p);
@@ -567,6 +574,17 @@ struct cr_server {
         uint64_t arrival_ns;
         Item item;
     };
+    // The state tables of one finished launch, kept in HBM for read-back:
+    // the launch slot's table buffer itself (the slot grows a new one).
+    struct Retained {
+        Buf buf;
+        uint32_t refs = 0;  // tickets not released yet
+        uint64_t seq = 0;   // completion order (oldest evicted first)
+    };
+    struct TableRef {
+        std::shared_ptr<Retained> owner;
+        uint64_t offset = 0, bytes = 0;
+    };
     struct Slot {
         Engine eng;
         bool busy = false;
@@ -581,12 +599,38 @@ struct cr_server {
     std::deque<Request> pending;
     std::deque<cr_graph_result> done;
     uint64_t next_ticket = 1, submitted = 0, finished = 0, launches = 0;
+    bool retain = false;                 // keep finished tables resident (cr_server_retain_tables)
+    std::unordered_map<uint64_t, TableRef> tables;  // ticket -> its resident table
+    std::deque<std::shared_ptr<Retained>> retained;  // in completion order
+    uint64_t retained_bytes = 0, retained_seq = 0, evicted = 0;
     bool stopping = false;
+
+    void drop_retained(const std::shared_ptr<Retained>& r);
+    void evict_for(uint64_t need);
     std::thread th;
     std::string thread_error;
 
     void run();
 };
+
+// Frees a retained launch's tables (every ticket of it loses its table).
+void cr_server::drop_retained(const std::shared_ptr<Retained>& r) {
+    for (auto it = tables.begin(); it != tables.end();)
+        it = it->second.owner == r ? tables.erase(it) : std::next(it);
+    retained_bytes -= r->buf.cap;
+    in_use -= r->buf.cap;
+    drop(r->buf);
+    retained.erase(std::remove(retained.begin(), retained.end(), r), retained.end());
+}
+
+// Retained tables live within the HBM budget: before admitting `need` more
+// bytes that would not fit, the oldest retained launches are evicted.
+void cr_server::evict_for(uint64_t need) {
+    while (!retained.empty() && in_use + need > budget) {
+        ++evicted;
+        drop_retained(retained.front());
+    }
+}
 
 void cr_server::run() {
     hipSetDevice(device);
@@ -599,15 +643,27 @@ void cr_server::run() {
             if (!s.busy || hipEventQuery(s.eng.ev1) != hipSuccess) continue;
             if (hipStreamQuery(s.eng.stream) != hipSuccess) continue;  // results still copying
             const uint64_t t = now_ns();
+            std::shared_ptr<Retained> keep;
+            if (retain && !s.reqs.empty()) {  // the slot's table buffer becomes the launch's resident tables
+                keep = std::make_shared<Retained>();
+                keep->buf = s.eng.d_tables;
+                s.eng.d_tables = Buf{};
+                keep->refs = static_cast<uint32_t>(s.reqs.size());
+                keep->seq = ++retained_seq;
+                retained.push_back(keep);
+                retained_bytes += keep->buf.cap;
+            }
             for (uint32_t i = 0; i < s.reqs.size(); ++i) {
                 cr_graph_result r;
                 s.eng.result(i, s.reqs[i].item, r);
                 r.ticket = s.reqs[i].ticket;
                 r.latency_ms = static_cast<double>(t - s.reqs[i].arrival_ns) / 1e6;
                 done.push_back(r);
+                if (keep) tables[r.ticket] = TableRef{keep, s.eng.graphs[i].table_off, s.reqs[i].item.table_bytes};
             }
             finished += s.reqs.size();
             in_use -= s.eng.footprint;
+            if (keep) in_use += keep->buf.cap;
             s.reqs.clear();
             s.busy = false;
             progress = true;
@@ -622,6 +678,7 @@ void cr_server::run() {
             while (!pending.empty() && static_cast<int32_t>(take.size()) < max_batch) {
                 const Item& it = pending.front().item;
                 const uint64_t need = it.table_bytes + it.slot_words * 8;
+                evict_for(bytes + need);
                 if (in_use + bytes + need > budget && !take.empty()) break;
                 if (in_use + bytes + need > budget && in_use > 0) break;  // wait for memory to free up
                 bytes += need;
@@ -898,6 +955,51 @@ int cr_server_stats(cr_server* s, uint64_t* submitted, uint64_t* finished, uint6
     return CR_OK;
 }
 
+int cr_server_retain_tables(cr_server* s, int32_t on) {
+    if (s == nullptr) return fail(CR_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->retain = on != 0;
+    return CR_OK;
+}
+
+int cr_server_copy_table(cr_server* s, uint64_t ticket, uint8_t* dst, uint64_t cap, uint64_t* bytes) {
+    if (s == nullptr) return fail(CR_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tables.find(ticket);
+    if (it == s->tables.end())
+        return fail(CR_ERR_ARG, "no resident table for ticket %llu (not finished, not retained, released or evicted)",
+                    static_cast<unsigned long long>(ticket));
+    const TableRefView ref{it->second.owner->buf.p, it->second.offset, it->second.bytes};
+    if (bytes) *bytes = ref.bytes;
+    if (dst == nullptr) return CR_OK;  // size query
+    if (cap < ref.bytes) return fail(CR_ERR_ARG, "table of ticket %llu has %llu bytes; buffer holds %llu",
+                                     static_cast<unsigned long long>(ticket), static_cast<unsigned long long>(ref.bytes),
+                                     static_cast<unsigned long long>(cap));
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpy(dst, static_cast<const uint8_t*>(ref.base) + ref.offset, ref.bytes, hipMemcpyDeviceToHost));
+    return CR_OK;
+}
+
+int cr_server_release_table(cr_server* s, uint64_t ticket) {
+    if (s == nullptr) return fail(CR_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tables.find(ticket);
+    if (it == s->tables.end()) return fail(CR_ERR_ARG, "no resident table for ticket %llu", static_cast<unsigned long long>(ticket));
+    std::shared_ptr<cr_server::Retained> owner = it->second.owner;
+    s->tables.erase(it);
+    if (--owner->refs == 0) s->drop_retained(owner);
+    return CR_OK;
+}
+
+int cr_server_table_stats(cr_server* s, uint64_t* resident_tables, uint64_t* resident_bytes, uint64_t* evicted_launches) {
+    if (s == nullptr) return fail(CR_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (resident_tables) *resident_tables = s->tables.size();
+    if (resident_bytes) *resident_bytes = s->retained_bytes;
+    if (evicted_launches) *evicted_launches = s->evicted;
+    return CR_OK;
+}
+
 void cr_server_destroy(cr_server* s) {
     if (!s) return;
     {
@@ -911,6 +1013,7 @@ void cr_server_destroy(cr_server* s) {
         if (sl->eng.stream) hipStreamSynchronize(sl->eng.stream);
         sl->eng.release();
     }
+    for (auto& r : s->retained) drop(r->buf);
     delete s;
 }
 

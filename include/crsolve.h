@@ -380,6 +380,20 @@ int cr_server_poll(cr_server* s, cr_graph_result* out, int32_t cap, int32_t wait
 int cr_server_stats(cr_server* s, uint64_t* submitted, uint64_t* finished, uint64_t* launches,
                     uint64_t* peak_bytes);
 /* Finishes the work in flight and stops the dispatcher. */
+/* Resident tables. Every reference solve emits its full state table
+ * (CacheManager.h:28-75); a server with retention on keeps the table of each
+ * request it finishes in HBM, in the reference's layout, until the caller
+ * releases it. Retained tables count against hbm_budget: when new work needs
+ * the room, the tables of the oldest finished launches are evicted (their
+ * tickets then have no table). */
+int cr_server_retain_tables(cr_server* s, int32_t on);
+/* Copies the table of a finished request to dst (cap bytes; dst NULL: only
+ * *bytes is set). CR_ERR_ARG when the ticket has no resident table. */
+int cr_server_copy_table(cr_server* s, uint64_t ticket, uint8_t* dst, uint64_t cap, uint64_t* bytes);
+/* Frees a request's table (its launch's memory once all its tables are released). */
+int cr_server_release_table(cr_server* s, uint64_t ticket);
+int cr_server_table_stats(cr_server* s, uint64_t* resident_tables, uint64_t* resident_bytes,
+                          uint64_t* evicted_launches);
 void cr_server_destroy(cr_server* s);
 /* Steady clock in nanoseconds (the arrival_ns time base). */
 uint64_t cr_clock_ns(void);

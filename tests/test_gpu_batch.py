@@ -234,3 +234,63 @@ def test_server_budget_admits_one_at_a_time_and_rejects_oversize(batch):
     assert need <= stats["peak_bytes"] <= need * 1.5
     for t in tickets:
         assert got[t]["table_hash"] == ref["table_hash"] and got[t]["plys"] == ref["plys"]
+
+
+def test_server_keeps_tables_resident_and_readable(batch):
+    """With retention on, each served request's table stays in HBM and reads
+    back byte-equal to the batched solve of the same graph (whose tables equal
+    the single-graph solver's, above); releasing frees them."""
+    graphs = workloads.batch_graphs(300, seed_base=4000)
+    want, tabs = batch.solve(graphs, tables=True)
+    say("server resident tables")
+    got = {}
+    with crb.Server(0, hbm_budget=8 << 30, max_batch=64, streams=2) as srv:
+        srv.retain_tables(True)
+        tickets, status = srv.submit_many(graphs)
+        assert all(t > 0 for t in tickets) and not any(status)
+        deadline = time.time() + 60
+        while len(got) < len(graphs) and time.time() < deadline:
+            for r in srv.poll(wait_us=2000):
+                got[r["ticket"]] = r
+        assert len(got) == len(graphs)
+        st = srv.table_stats()
+        assert st["resident_tables"] == len(graphs) and st["evicted_launches"] == 0
+        for i, t in enumerate(tickets):
+            assert got[t]["table_hash"] == want[i]["table_hash"]
+            assert np.array_equal(srv.table(t), tabs[i]), i
+        for t in tickets[::2]:
+            srv.release(t)
+        with pytest.raises(cra.SolverError):
+            srv.table(tickets[0])
+        assert np.array_equal(srv.table(tickets[1]), tabs[1])
+        for t in tickets[1::2]:
+            srv.release(t)
+        st = srv.table_stats()
+        assert st["resident_tables"] == 0 and st["resident_bytes"] == 0
+
+
+def test_server_evicts_oldest_tables_within_the_budget(batch):
+    adj, _ = workloads.er_graph(30, 0.15, 4)
+    graphs = [(adj, 3)] * 10
+    ref, tabs = batch.solve(graphs[:1], tables=True)
+    need = request_bytes(30, 3)
+    say("server table eviction", need)
+    got = {}
+    with crb.Server(0, hbm_budget=int(need * 2.5), max_batch=1, streams=1) as srv:
+        srv.retain_tables(True)
+        tickets = []
+        for a, k in graphs:  # one at a time, so each launch retains one table
+            tickets.append(srv.submit(a, k))
+            deadline = time.time() + 30
+            while tickets[-1] not in got and time.time() < deadline:
+                for r in srv.poll(wait_us=2000):
+                    got[r["ticket"]] = r
+        st = srv.table_stats()
+        assert len(got) == len(graphs)
+        assert st["evicted_launches"] >= 1
+        assert np.array_equal(srv.table(tickets[-1]), tabs[0])
+        with pytest.raises(cra.SolverError):
+            srv.table(tickets[0])
+        assert st["resident_bytes"] <= need * 2.5
+    for t in tickets:
+        assert got[t]["table_hash"] == ref[0]["table_hash"]
