@@ -1031,7 +1031,7 @@ typedef struct {
     uint8_t* escapes;
     unsigned depth;
     double deadline;
-    volatile int* stop;
+    int* stop;          /* set (atomically) by the first thread past the deadline */
     uint64_t* next_row; /* shared chunk counter */
     int phase;          /* 0 = init, 1 = level */
 } cap_shared;
@@ -1055,7 +1055,7 @@ static void* cap_run(void* arg) {
     uint8_t cops[OR_MAX_K];
     const uint8_t nd = WON_BYTE(s->depth + 1);
     for (;;) {
-        if (*s->stop) break;
+        if (__atomic_load_n(s->stop, __ATOMIC_RELAXED)) break;
         uint64_t c0 = __atomic_fetch_add(s->next_row, CAP_CHUNK, __ATOMIC_RELAXED);
         if (c0 >= s->C) break;
         uint64_t c1 = c0 + CAP_CHUNK < s->C ? c0 + CAP_CHUNK : s->C;
@@ -1101,7 +1101,7 @@ static void* cap_run(void* arg) {
             }
         }
         w->rows += c1 - c0;
-        if (now_s() > s->deadline) *s->stop = 1;
+        if (now_s() > s->deadline) __atomic_store_n(s->stop, 1, __ATOMIC_RELAXED);
     }
     free(buf);
     return NULL;
@@ -1125,13 +1125,13 @@ int32_t or_v1_levelsync_capped(int32_t n, const uint8_t* adj, int32_t k, int32_t
     pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
     if (!esc || !ws || !th) { free(g); free(esc); free(ws); free(th); return -1; }
     double t0 = now_s();
-    volatile int stop = 0;
+    int stop = 0;
     uint64_t ec = 0, er = 0;
     int done = 0, err = 0;
     double frac = 0.0;
     /* phase 0 = the captures (initializeCaptures, :85-127), then one phase
      * per depth until a level marks nothing or the cap stops it */
-    for (int phase = 0; !stop; ++phase) {
+    for (int phase = 0; !__atomic_load_n(&stop, __ATOMIC_RELAXED); ++phase) {
         unsigned depth = phase == 0 ? 0 : (unsigned)(phase - 1);
         if (phase > 0 && depth + 1 >= OR_MAX_ROUND) break;
         uint64_t next = 0;
