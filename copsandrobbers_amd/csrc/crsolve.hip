@@ -1285,10 +1285,18 @@ __global__ __launch_bounds__(kBlock) void k_clear_frontier(LevelArgs a, int32_t 
 // robber r is won iff every move of r lands on a won cop-turn state
 // (out_mask[r] within won0); depth = d + 1. Lanes = entries of list0.
 // ---------------------------------------------------------------------------
+// Robber positions that can stay open in a row, OR_{v open} in_mask[v], by
+// nibbles of the open set: lds_nib[(p * 16 + x) * W + q] = word q of the OR of
+// the in-masks of the vertices 4p + j for the set bits j of x. A row then
+// costs one table row per nibble (n / 4) whatever its density.
+constexpr int kNibbles = (CR_MAX_N + 3) / 4;
+constexpr uint32_t kNibbleMin = 24;  // open states above which a row takes the nibble table
+
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs a) {
     __shared__ uint64_t lds_om[CR_MAX_N * W];
     __shared__ uint64_t lds_im[CR_MAX_N * W];
+    __shared__ uint64_t lds_nib[kNibbles * 16 * W];
     const int lane = threadIdx.x & 63;
     if (a.ctl->done) return;
     const uint32_t level = a.ctl->level, cur = level & 1;
@@ -1299,6 +1307,21 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
     if (blockIdx.x >= ngroups) return;  // no rows here: skip the mask staging (uniform)
     const uint64_t* lom = stage_out_masks<W>(lds_om, a.out_mask, gv.n);
     const uint64_t* lim = stage_out_masks<W>(lds_im, a.in_mask, gv.n);
+    const int nnib = (gv.n + 3) / 4;
+    for (int e = threadIdx.x; e < nnib * 16; e += blockDim.x) {
+        const int p = e >> 4, x = e & 15;
+        uint64_t acc[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) acc[q] = 0;
+        for (int j = 0; j < 4; ++j)
+            if (((x >> j) & 1) && 4 * p + j < gv.n) {
+#pragma unroll
+                for (int q = 0; q < W; ++q) acc[q] |= lim[(4 * p + j) * W + q];
+            }
+#pragma unroll
+        for (int q = 0; q < W; ++q) lds_nib[e * W + q] = acc[q];
+    }
+    __syncthreads();
     const uint32_t wid = wave_id();
     const int n = gv.n;
     const uint8_t nb = won_byte(level + 1);
@@ -1349,22 +1372,36 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
                 nopen += __popcll(open[q]);
                 nnew += __popcll(f0[q]);
             }
-            if (nopen <= a.open_factor * nnew) {
-                // Few cop-turn states of the row are still open: a robber with a
-                // move onto an open state stays open, and every other robber not
-                // yet won is won at this depth (had all its moves been won by an
-                // earlier depth, an earlier level would have taken it).
+            const bool by_open = nopen <= a.open_factor * nnew;
+            const bool by_nibble = !by_open && nopen > kNibbleMin && nnew > kNibbleMin / 4;
+            if (by_open || by_nibble) {
+                // A robber with a move onto an open cop-turn state stays open, and
+                // every other robber not yet won is won at this depth (had all its
+                // moves been won by an earlier depth, an earlier level would have
+                // taken it). The open set is walked bit by bit where it is small
+                // and nibble by nibble through the table where it is not.
                 uint64_t stay[W];
 #pragma unroll
                 for (int q = 0; q < W; ++q) stay[q] = 0;
+                if (by_open && nopen <= kNibbleMin) {
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    uint64_t x = open[w];
-                    while (x) {
-                        const uint64_t* im = lim + (64 * w + __builtin_ctzll(x)) * W;
-                        x &= x - 1;
+                    for (int w = 0; w < W; ++w) {
+                        uint64_t x = open[w];
+                        while (x) {
+                            const uint64_t* im = lim + (64 * w + __builtin_ctzll(x)) * W;
+                            x &= x - 1;
 #pragma unroll
-                        for (int q = 0; q < W; ++q) stay[q] |= im[q];
+                            for (int q = 0; q < W; ++q) stay[q] |= im[q];
+                        }
+                    }
+                } else {
+                    for (int p = 0; p < nnib; ++p) {
+                        const uint32_t x = static_cast<uint32_t>(open[p >> 4] >> (4 * (p & 15))) & 15u;
+                        if (x == 0) continue;
+                        uint64_t v[W];
+                        load_row<W>(lds_nib + (p * 16 + x) * W, v);
+#pragma unroll
+                        for (int q = 0; q < W; ++q) stay[q] |= v[q];
                     }
                 }
 #pragma unroll
