@@ -1879,12 +1879,13 @@ struct LogList {
     uint32_t turn_byte;        // turn << 8 | depth byte
 };
 
-// Per chunk c and list L: cnt[L * nchunks + c] entries of L have rows in
-// chunk c, the first at beg[L * nchunks + c] (cnt is zeroed before; beg is
-// only read where cnt > 0). One thread per entry; the first entry of a chunk finds the chunk's
-// last one by binary search over at most R entries.
+// Per chunk c and list L: the entries of L with rows in chunk c are
+// [beg, end) at index L * nchunks + c (end is zeroed before; beg is only read
+// where end > 0). One thread per entry; the lists are sorted by row, so the
+// first entry of a chunk writes its beg and the last its end (the
+// neighbouring entries' chunks decide).
 __global__ __launch_bounds__(kBlock) void k_log_chunks(const LogList* lists, int32_t nl, uint64_t total,
-                                                       const uint32_t* rows, uint32_t* beg, uint32_t* cnt,
+                                                       const uint32_t* rows, uint32_t* beg, uint32_t* end,
                                                        uint32_t R, uint32_t nchunks) {
     __shared__ LogList sl[256];
     for (int L = threadIdx.x; L < nl; L += blockDim.x) sl[L] = lists[L];
@@ -1900,15 +1901,9 @@ __global__ __launch_bounds__(kBlock) void k_log_chunks(const LogList* lists, int
         const uint32_t i = static_cast<uint32_t>(g - ll.first);
         const uint32_t* lr = rows + ll.off;
         const uint32_t c = lr[i] / R;
-        if (i > 0 && lr[i - 1] / R == c) continue;  // not the chunk's first entry
-        uint32_t a = i + 1, b = min(ll.len, i + R);  // first entry past the chunk lies in [a, b]
-        while (a < b) {
-            const uint32_t mid = (a + b) >> 1;
-            if (lr[mid] / R == c) a = mid + 1; else b = mid;
-        }
-        // list-major: consecutive chunk-first entries of a list store side by side
-        beg[static_cast<uint64_t>(lo) * nchunks + c] = i;
-        cnt[static_cast<uint64_t>(lo) * nchunks + c] = a - i;
+        // list-major: consecutive chunks of a list store side by side
+        if (i == 0 || lr[i - 1] / R != c) beg[static_cast<uint64_t>(lo) * nchunks + c] = i;
+        if (i + 1 == ll.len || lr[i + 1] / R != c) end[static_cast<uint64_t>(lo) * nchunks + c] = i + 1;
     }
 }
 
@@ -1942,7 +1937,7 @@ __device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t& total) 
 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_apply_log(const LogList* lists, int32_t nl, const uint32_t* rows,
-                                                      const uint64_t* bits, const uint32_t* beg, const uint32_t* cnt,
+                                                      const uint64_t* bits, const uint32_t* beg, const uint32_t* end,
                                                       uint8_t* table, uint32_t n, uint32_t C, uint32_t R) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint32_t s_beg[256], s_pre[256];
@@ -1959,8 +1954,12 @@ __global__ __launch_bounds__(kBlock) void k_apply_log(const LogList* lists, int3
         __syncthreads();  // the previous chunk is written back
         uint32_t c = 0;
         if (threadIdx.x < static_cast<uint32_t>(nl)) {
-            c = cnt[static_cast<uint64_t>(threadIdx.x) * nchunks + ch];
-            if (c) s_beg[threadIdx.x] = beg[static_cast<uint64_t>(threadIdx.x) * nchunks + ch];
+            const uint32_t e = end[static_cast<uint64_t>(threadIdx.x) * nchunks + ch];
+            if (e) {
+                const uint32_t b = beg[static_cast<uint64_t>(threadIdx.x) * nchunks + ch];
+                s_beg[threadIdx.x] = b;
+                c = e - b;
+            }
         }
         uint32_t total = 0;
         const uint32_t pre = block_scan_u32(c, total);
@@ -3058,16 +3057,16 @@ struct LaunchApplyLog {
         const uint32_t nchunks = static_cast<uint32_t>((s->C + R - 1) / R);
         const uint64_t cells = static_cast<uint64_t>(nchunks) * nl;
         uint32_t* beg = s->d_starts;
-        uint32_t* cnt = s->d_starts + static_cast<uint64_t>(nchunks) * kMaxLogLists;
+        uint32_t* end = s->d_starts + static_cast<uint64_t>(nchunks) * kMaxLogLists;
         hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<uint32_t>(std::min<uint64_t>((cells + kBlock - 1) / kBlock, 4096))),
-                           dim3(kBlock), 0, s->stream, cnt, cells, 0u);
+                           dim3(kBlock), 0, s->stream, end, cells, 0u);
         const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((total + kBlock - 1) / kBlock, 8192)));
         hipLaunchKernelGGL(k_log_chunks, dim3(sgrid), dim3(kBlock), 0, s->stream, s->d_lists, nl, total, s->d_log_rows,
-                           beg, cnt, R, nchunks);
+                           beg, end, R, nchunks);
         const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nchunks, 65536));
         if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_apply_log<W>), lds)) return rc;
         hipLaunchKernelGGL(k_apply_log<W>, dim3(grid), dim3(kBlock), lds, s->stream, s->d_lists, nl, s->d_log_rows,
-                           s->d_log_bits, beg, cnt, s->d_table, n, static_cast<uint32_t>(s->C), R);
+                           s->d_log_bits, beg, end, s->d_table, n, static_cast<uint32_t>(s->C), R);
         return 0;
     }
 };
