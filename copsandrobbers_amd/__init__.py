@@ -24,7 +24,8 @@ TUNING = {"push_filter": 1, "push_blocks": 2, "row_blocks": 3, "levels_per_sync"
           "pull_scale": 6, "pull_tiles": 7, "pull_tile_rows": 8,
           "pull_tile_blocks": 9, "depth_log": 10, "depth_log_min": 11,
           "pull_tile_chunk": 12, "fuse_finalize": 13, "pull_tile1": 14, "pull_gate": 15, "robber_open": 16, "log_blocks": 17, "scan_tile": 18, "push_dirty": 19,
-          "pull_pair": 20, "pair_rows": 21, "pair_blocks": 22}
+          "pull_pair": 20, "pair_rows": 21, "pair_blocks": 22,
+          "end_pass": 23}
 
 STATUS = {
     0: "CR_OK", -1: "CR_ERR_ARG", -2: "CR_ERR_IO", -3: "CR_ERR_EMPTY_GRAPH",

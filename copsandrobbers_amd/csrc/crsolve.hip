@@ -94,6 +94,13 @@ struct Ctl {
     unsigned long long hist[2][128];  // retro-edges by depth: [0] cop (robber states), [1] robber (cop states)
     unsigned long long verdict;
     unsigned long long retro[3];
+    // k_end_level (single-pass level end): per level, the tile tickets and the
+    // spaces that have finished; per space, the states of its tiles and its total
+    uint32_t lb_ticket[128];
+    uint32_t lb_arrived[128];
+    unsigned long long lb_states[2];
+    uint32_t lb_total[2];
+    uint32_t done_level;  // the level whose end set `done`
 };
 
 // Row sets are compacted without contended atomics. A producer kernel walks an
@@ -145,6 +152,8 @@ struct LevelArgs {
     uint32_t tile_stride;                 // entries per list in tile_cnt / tile_states
     uint32_t* tile_cnt;                   // [2][tile_stride]: tile totals, then offsets
     unsigned long long* tile_states;      // [2][tile_stride]
+    unsigned long long* lb_status;        // k_end_level: [2][lb_stride] published tile sums (epoch, flag, value)
+    uint32_t lb_stride;
 };
 
 // Work items [0, n) in 8 contiguous segments, one per XCD (workgroups are
@@ -1665,13 +1674,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(LevelArgs a, uint32
     }
 }
 
+__device__ void finish_level(const LevelArgs& a, uint32_t d, uint32_t t0, uint32_t t1, unsigned long long s0,
+                             unsigned long long s1);
+
 __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
     Ctl* ctl = a.ctl;
     if (ctl->done) {  // a queued level after the last one: its k_write_lists does nothing
         if (threadIdx.x == 0) ctl->log_open = 0;
         return;
     }
-    const uint32_t d = ctl->level, cur = d & 1, nxt = cur ^ 1;
+    const uint32_t d = ctl->level, cur = d & 1;
     const uint32_t ng0 = (a.nbatch + kGroupWords - 1) / kGroupWords;
     const uint32_t nw1 = (ctl->cnt0[cur] + 63) / 64;
     const uint32_t ng1 = (nw1 + kGroupWords - 1) / kGroupWords;
@@ -1688,9 +1700,16 @@ __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
         const uint32_t ng[2] = {ng0, ng1};
         scan_groups_n<2>(gc, gs, ng, tt, ss);
     }
-    const uint32_t t0 = tt[0], t1 = tt[1];
-    const unsigned long long s0 = ss[0], s1 = ss[1];
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0) finish_level(a, d, tt[0], tt[1], ss[0], ss[1]);
+}
+
+// The end of level d once both lists are counted: histogram, list lengths,
+// direction and log room of depth d + 1, and the stop test.
+__device__ void finish_level(const LevelArgs& a, uint32_t d, uint32_t t0, uint32_t t1, unsigned long long s0,
+                             unsigned long long s1) {
+    Ctl* ctl = a.ctl;
+    const uint32_t nxt = (d & 1) ^ 1;
+    {
         ctl->lvl[d + 1][0] = s0;
         ctl->lvl[d + 1][1] = s1;
         ctl->cnt0[nxt] = t0;
@@ -1710,6 +1729,7 @@ __global__ __launch_bounds__(kScanThreads) void k_advance(LevelArgs a) {
         if (ctl->log_now) ctl->log_used += need;
         if ((t0 == 0 && t1 == 0) || d + 1 >= kMaxRound - 1) {
             ctl->done = 1;
+            ctl->done_level = d;
             ctl->log_open = 1;  // the depth d + 1 lists still carry their states' bytes
         }
     }
@@ -1746,6 +1766,44 @@ __global__ __launch_bounds__(kScanThreads) void k_settle_init(LevelArgs a) {
         ctl->dir[1] = ctl->pull;
         ctl->level = 1;
         ctl->done = (t0 == 0 && t1 == 0) ? 1u : 0u;
+    }
+}
+
+// The list entries of one flag word per lane (consecutive words in
+// consecutive lanes; every lane of the wave calls it): bit j of word w is
+// entry pos + (set bits below j), holding w * 64 + j, or map[w * 64 + j] when
+// the word indexes another list. Small tables and sparse words: each lane
+// writes its own word's entries; dense words in HBM: the wave writes them one
+// word at a time, lane j the entry of bit j, so stores (and the map reads) are
+// contiguous.
+__device__ __forceinline__ void write_word_entries(uint64_t bits, uint32_t pc, uint32_t pos, uint32_t word,
+                                                   uint32_t* dst, const uint32_t* map, bool small) {
+    const uint32_t lane = threadIdx.x & 63;
+    unsigned long long live = __ballot(bits != 0);
+    if (!live) return;  // uniform
+    uint32_t maxpc = pc;  // the wave's densest word
+    for (int off = 32; off > 0; off >>= 1) maxpc = max(maxpc, static_cast<uint32_t>(__shfl_xor(static_cast<int>(maxpc), off)));
+    if (small || 4 * maxpc <= static_cast<uint32_t>(__popcll(live))) {
+        uint32_t p = pos;
+        uint64_t rest = bits;
+        while (rest) {
+            const uint32_t idx = word * 64 + static_cast<uint32_t>(__builtin_ctzll(rest));
+            rest &= rest - 1;
+            dst[p++] = map ? map[idx] : idx;
+        }
+        return;
+    }
+    while (live) {
+        const int w = __builtin_ctzll(live);
+        live &= live - 1;
+        const uint32_t lo = __shfl(static_cast<uint32_t>(bits), w);
+        const uint32_t hi = __shfl(static_cast<uint32_t>(bits >> 32), w);
+        const uint32_t wpos = __shfl(pos, w);
+        const uint64_t wb = (static_cast<uint64_t>(hi) << 32) | lo;
+        if ((wb >> lane) & 1ull) {
+            const uint32_t idx = (word - lane + w) * 64 + lane;
+            dst[wpos + __popcll(wb & ((1ull << lane) - 1))] = map ? map[idx] : idx;
+        }
     }
 }
 
@@ -1787,38 +1845,124 @@ __global__ __launch_bounds__(kBlock) void k_write_lists(LevelArgs a, int32_t ini
             if (a.scan_tile && !init_mode) pos += a.tile_cnt[(space1 ? a.tile_stride : 0u) + g / a.scan_tile];
         }
         uint32_t* dst = space1 ? a.list1[nxt] : a.list0[nxt];
-        unsigned long long live = __ballot(bits != 0);
-        if (!live) continue;  // uniform
-        uint32_t maxpc = pc;  // the wave's densest word
-        for (int off = 32; off > 0; off >>= 1) maxpc = max(maxpc, static_cast<uint32_t>(__shfl_xor(static_cast<int>(maxpc), off)));
-        if (small || 4 * maxpc <= static_cast<uint32_t>(__popcll(live))) {
-            // lists that stay in the caches (the depth log is kept from 2^24
-            // configurations on), or sparse words (in HBM a scattered store
-            // costs several contiguous ones): each lane writes its own word's
-            // entries
-            uint32_t p = pos;
-            uint64_t rest = bits;
-            while (rest) {
-                const uint32_t idx = word * 64 + static_cast<uint32_t>(__builtin_ctzll(rest));
-                rest &= rest - 1;
-                dst[p++] = (space1 && !init_mode) ? a.list0[prev][idx] : idx;
+        write_word_entries(bits, pc, pos, word, dst, (space1 && !init_mode) ? a.list0[prev] : nullptr, small);
+    }
+}
+
+// Single-pass end of level d for tables whose group counts one scan covers
+// (no two-level scan): k_advance's scan and bookkeeping and k_write_lists in
+// one launch. Workgroups take tiles of kEndTileGroups groups (a flag word per
+// thread) in ticket order, the cop-turn space first. A tile publishes its
+// count, then walks back over its predecessors' published sums (a
+// predecessor's inclusive total ends the walk, its count alone does not:
+// decoupled look-back), publishes its inclusive total and writes its words'
+// list entries. A tile only waits on tiles with smaller tickets, which are
+// running or done. The last tile of each space leaves the space's total; the
+// later of the two ends the level. `d` comes from the host: levels enqueued
+// together advance one each, and a workgroup that starts after the level has
+// ended must not read the next level's number.
+constexpr uint32_t kEndTileGroups = kBlock / kGroupWords;
+
+__global__ __launch_bounds__(kBlock) void k_end_level(LevelArgs a, uint32_t d) {
+    Ctl* ctl = a.ctl;
+    if (ctl->done) {
+        // a level queued after the last one (the last level's own late
+        // workgroups find its tiles taken and leave the log flag alone)
+        if (d > ctl->done_level && blockIdx.x == 0 && threadIdx.x == 0) ctl->log_open = 0;
+        return;
+    }
+    if (d >= kMaxRound) return;  // never reached before `done` (levels run d = 1..125)
+    const uint32_t cur = d & 1, nxt = cur ^ 1;
+    const uint32_t nw0 = a.nbatch, nw1 = (ctl->cnt0[cur] + 63) / 64;
+    const uint32_t ng0 = (nw0 + kGroupWords - 1) / kGroupWords, ng1 = (nw1 + kGroupWords - 1) / kGroupWords;
+    const uint32_t T0 = (ng0 + kEndTileGroups - 1) / kEndTileGroups, T1 = (ng1 + kEndTileGroups - 1) / kEndTileGroups;
+    const bool small = a.log_rows == nullptr;
+    const unsigned long long epoch = static_cast<unsigned long long>(d + 1) << 40;
+    __shared__ uint32_t s_tile, s_base;
+    __shared__ uint32_t s_gpre[kEndTileGroups];
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&ctl->lb_ticket[d], 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= T0 + T1) break;  // uniform
+        const uint32_t sp = tile >= T0 ? 1u : 0u, tt = sp ? tile - T0 : tile;
+        const Compact& c = sp ? a.c1 : a.c0;
+        const uint32_t ng = sp ? ng1 : ng0, nw = sp ? nw1 : nw0, T = sp ? T1 : T0;
+        const uint32_t g0 = tt * kEndTileGroups;
+        if (wid == 0) {  // the tile's group counts: in-tile offsets, count, states; then the look-back
+            const uint32_t g = g0 + lane;
+            const uint32_t cg = g < ng ? c.gcnt[g] : 0u;
+            unsigned long long sg = g < ng ? c.gstates[g] : 0ull;
+            uint32_t incl = cg;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t u = __shfl_up(incl, off);
+                if (lane >= static_cast<uint32_t>(off)) incl += u;
             }
-            continue;
-        }
-        // dense words: the wave writes them one at a time, lane j the entry of
-        // bit j, so stores (and the list0 reads of space 1) are contiguous
-        while (live) {
-            const int w = __builtin_ctzll(live);
-            live &= live - 1;
-            const uint32_t lo = __shfl(static_cast<uint32_t>(bits), w);
-            const uint32_t hi = __shfl(static_cast<uint32_t>(bits >> 32), w);
-            const uint32_t wpos = __shfl(pos, w);
-            const uint64_t wb = (static_cast<uint64_t>(hi) << 32) | lo;
-            if ((wb >> lane) & 1ull) {
-                const uint32_t idx = (word - lane + w) * 64 + lane;
-                dst[wpos + __popcll(wb & ((1ull << lane) - 1))] = (space1 && !init_mode) ? a.list0[prev][idx] : idx;
+            s_gpre[lane] = incl - cg;
+            const uint32_t agg = __shfl(incl, 63);
+            sg = wave_sum(sg);
+            unsigned long long* st = a.lb_status + static_cast<uint64_t>(sp) * a.lb_stride;
+            if (lane == 0 && sg) atomicAdd(&ctl->lb_states[sp], sg);  // before the count is published
+            if (lane == 0 && tt > 0)
+                __hip_atomic_store(&st[tt], epoch | (1ull << 32) | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            // look-back, 64 predecessors per step: lane j reads tile p_hi - j; the walk
+            // sums back to the nearest inclusive total once every tile up to it has published
+            uint32_t prefix = 0;
+            for (int32_t p_hi = static_cast<int32_t>(tt) - 1; p_hi >= 0;) {
+                const int32_t p = p_hi - static_cast<int32_t>(lane);
+                unsigned long long v = 0;
+                if (p >= 0) v = __hip_atomic_load(&st[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const bool ok = p < 0 || ((v >> 40) == d + 1 && ((v >> 32) & 0xffull) != 0);
+                const bool inc = p >= 0 && ok && ((v >> 32) & 0xffull) == 2;
+                const unsigned long long incl_mask = __ballot(inc), ready = __ballot(ok);
+                const uint32_t stop = incl_mask ? static_cast<uint32_t>(__builtin_ctzll(incl_mask)) : 63u;
+                const unsigned long long need = stop >= 63 ? ~0ull : ((2ull << stop) - 1);  // lanes 0..stop
+                if ((ready & need) != need) {  // a tile up to the stop has not published yet
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                const uint32_t part = (lane <= stop && p >= 0) ? static_cast<uint32_t>(v) : 0u;
+                prefix += static_cast<uint32_t>(wave_sum(static_cast<unsigned long long>(part)));
+                if (incl_mask) break;
+                p_hi -= 64;
+            }
+            if (lane == 0) {
+                __hip_atomic_store(&st[tt], epoch | (2ull << 32) | (prefix + agg), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                s_base = prefix;
+                if (tt + 1 == T) {  // the space's last tile: its total; the second space to finish ends the level
+                    __hip_atomic_store(&ctl->lb_total[sp], prefix + agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __threadfence();
+                    const uint32_t expected = (T0 > 0 ? 1u : 0u) + (T1 > 0 ? 1u : 0u);
+                    if (atomicAdd(&ctl->lb_arrived[d], 1u) + 1 == expected) {
+                        __threadfence();
+                        auto ld32 = [](uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+                        auto ld64 = [](unsigned long long* p) {
+                            return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        };
+                        const uint32_t t0 = T0 ? ld32(&ctl->lb_total[0]) : 0u, t1 = T1 ? ld32(&ctl->lb_total[1]) : 0u;
+                        const unsigned long long s0 = ld64(&ctl->lb_states[0]), s1 = ld64(&ctl->lb_states[1]);
+                        ctl->lb_states[0] = 0;
+                        ctl->lb_states[1] = 0;
+                        finish_level(a, d, t0, t1, s0, s1);
+                    }
+                }
             }
         }
+        __syncthreads();
+        // the tile's words, one per thread, to their list positions
+        const uint32_t word = g0 * kGroupWords + threadIdx.x;
+        const uint64_t bits = word < nw ? c.flags[word] : 0ull;
+        const uint32_t pc = __popcll(bits);
+        uint32_t incl = pc;
+        for (int off = 1; off < kGroupWords; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off, kGroupWords);
+            if (static_cast<int>(word % kGroupWords) >= off) incl += v;
+        }
+        const uint32_t pos = s_base + s_gpre[threadIdx.x / kGroupWords] + incl - pc;
+        write_word_entries(bits, pc, pos, word, sp ? a.list1[nxt] : a.list0[nxt], sp ? a.list0[cur] : nullptr, small);
+        __syncthreads();  // s_tile, s_base and s_gpre are rewritten by the next tile
     }
 }
 
@@ -2332,6 +2476,9 @@ struct cr_solver {
     uint32_t tile_stride = 0;
     uint32_t* d_tile_cnt = nullptr;
     uint8_t* d_dirty = nullptr;               // per group: marked by the push for k_cop_finalize
+    unsigned long long* d_lb_status = nullptr;  // k_end_level: published tile sums, [2][lb_stride]
+    uint32_t lb_stride = 0;
+    bool end_pass = true;                     // tuning: one k_end_level instead of k_advance + k_write_lists
     bool push_dirty = true;
     unsigned long long* d_tile_states = nullptr;
     uint32_t pull_gate = 4;                   // a chunk may pull if its first frontier * pull_gate > pull_rows
@@ -2404,6 +2551,8 @@ struct cr_solver {
         hipFree(d_tile_cnt);
         hipFree(d_dirty);
         d_dirty = nullptr;
+        hipFree(d_lb_status);
+        d_lb_status = nullptr;
         if (d_pair) hipFree(d_pair);
         d_pair = nullptr;
         pair_cap = 0;
@@ -2809,6 +2958,8 @@ LevelArgs level_args(cr_solver* s) {
     a.tile_cnt = s->d_tile_cnt;
     a.tile_states = s->d_tile_states;
     a.dirty = s->push_dirty ? s->d_dirty : nullptr;
+    a.lb_status = s->d_lb_status;
+    a.lb_stride = s->lb_stride;
     return a;
 }
 
@@ -3236,6 +3387,8 @@ int do_build(cr_solver* s) {
     s->tile_stride = static_cast<uint32_t>(ngroups / kScanTileMin + 1);  // tiles at the smallest tile size
     if (!(s->d_tile_cnt = dev_alloc<uint32_t>(2ull * s->tile_stride, s->device_bytes, rc))) return rc;
     if (!(s->d_dirty = dev_alloc<uint8_t>(ngroups, s->device_bytes, rc))) return rc;
+    s->lb_stride = static_cast<uint32_t>(ngroups / kEndTileGroups + 1);
+    if (!(s->d_lb_status = dev_alloc<unsigned long long>(2ull * s->lb_stride, s->device_bytes, rc))) return rc;
     if (!(s->d_tile_states = dev_alloc<unsigned long long>(2ull * s->tile_stride, s->device_bytes, rc))) return rc;
     if (!(s->d_ctl = dev_alloc<Ctl>(1, s->device_bytes, rc))) return rc;
     if (!s->h_ctl) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->h_ctl), sizeof(Ctl), hipHostMallocDefault));
@@ -3254,6 +3407,8 @@ int reset_ctl(cr_solver* s) {
     s->h_ctl->log_cap = s->log_cap;
     s->log_first = 0;
     HIP_TRY(hipMemcpyAsync(s->d_ctl, s->h_ctl, sizeof(Ctl), hipMemcpyHostToDevice, s->stream));
+    if (s->d_lb_status)  // tile sums of the last solve carry the same level epochs
+        HIP_TRY(hipMemsetAsync(s->d_lb_status, 0, 2ull * s->lb_stride * sizeof(unsigned long long), s->stream));
     return 0;
 }
 
@@ -3442,6 +3597,7 @@ int do_loop(cr_solver* s) {
             pulls = h->pull != 0 || frontier * static_cast<uint64_t>(s->pull_gate) > s->pull_rows;
         }
         if (!pulls) la.pull_rows = 0xffffffffu;
+        const uint32_t first_level = s->h_ctl->level;  // each enqueued level advances one (or returns after the last)
         for (int l = 0; l < s->levels_per_sync; ++l) {
             // The robber step (depth d + 1 robber-turn states from cop-turn
             // states of depth <= d) and the cop step (depth d + 1 cop-turn
@@ -3470,12 +3626,17 @@ int do_loop(cr_solver* s) {
             t = start_timer(s, "cop_finalize");
             if ((rc = dispatch_w<LaunchFinalize>(s->W, s, la)) != 0) return rc;
             t->stop();
-            if (la.scan_tile) {
-                const uint32_t T = static_cast<uint32_t>((s->ngroups() + la.scan_tile - 1) / la.scan_tile);
-                hipLaunchKernelGGL(k_scan_tiles, dim3(2 * T), dim3(kScanThreads), 0, s->stream, la, T);
+            if (!la.scan_tile && s->end_pass) {  // one pass: scan, level end and lists
+                const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(2ull * s->lb_stride, 1024));
+                hipLaunchKernelGGL(k_end_level, dim3(grid), dim3(kBlock), 0, s->stream, la, first_level + l);
+            } else {
+                if (la.scan_tile) {
+                    const uint32_t T = static_cast<uint32_t>((s->ngroups() + la.scan_tile - 1) / la.scan_tile);
+                    hipLaunchKernelGGL(k_scan_tiles, dim3(2 * T), dim3(kScanThreads), 0, s->stream, la, T);
+                }
+                hipLaunchKernelGGL(k_advance, dim3(1), dim3(kScanThreads), 0, s->stream, la);
+                hipLaunchKernelGGL(k_write_lists, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, la, 0);
             }
-            hipLaunchKernelGGL(k_advance, dim3(1), dim3(kScanThreads), 0, s->stream, la);
-            hipLaunchKernelGGL(k_write_lists, dim3(s->writer_grid()), dim3(kBlock), 0, s->stream, la, 0);
             if (s->log_cap && (rc = dispatch_w<LaunchLogLists>(s->W, s, la)) != 0) return rc;
         }
         HIP_TRY(hipGetLastError());
@@ -4016,6 +4177,7 @@ int cr_set_tuning(cr_solver* s, int32_t key, int64_t value) {
             return 0;
         case CR_TUNE_FUSE_FINALIZE: s->fuse_finalize = value != 0; return 0;
         case CR_TUNE_PUSH_DIRTY: s->push_dirty = value != 0; return 0;
+        case CR_TUNE_END_PASS: s->end_pass = value != 0; return 0;
         case CR_TUNE_PULL_PAIR:
             if (value < 0 || value > 2) return fail(CR_ERR_ARG, "pull pair %lld", static_cast<long long>(value));
             s->pull_pair = static_cast<int32_t>(value);

@@ -264,7 +264,10 @@ enum cr_tuning {
                                     next-to-last table in LDS: 0 never, 1 (default) for k = 3 (measured
                                     faster there; no gain at k = 4), 2 for every k >= 3 */
     CR_TUNE_PAIR_ROWS = 21,      /* cop multisets per tile of that kernel, 1..8 (default 4) */
-    CR_TUNE_PAIR_BLOCKS = 22     /* workgroups of that kernel, rounded to a multiple of 8 (default 1024) */
+    CR_TUNE_PAIR_BLOCKS = 22,    /* workgroups of that kernel, rounded to a multiple of 8 (default 1024) */
+    CR_TUNE_END_PASS = 23        /* 1 (default): where one scan covers a level's group counts (up to 8192
+                                    groups), the level ends in one launch (single-pass scan with
+                                    decoupled look-back, list writes); 0: a scan launch, then the lists */
 };
 int cr_set_tuning(cr_solver* s, int32_t key, int64_t value);
 /* Per-kernel device time of the last cr_main_loop when timing is enabled:

@@ -225,16 +225,19 @@ PULL_SETTINGS = [{"pull_scale": 0, "pull_tiles": 1, "pull_tile_rows": 4},
                  {"push_filter": 0, "pull_scale": 10**9},
                  {"push_blocks": 8, "pull_scale": 10**9},
                  {"pull_scale": 0, "pull_pair": 0},
+                 {"end_pass": 0, "levels_per_sync": 3},
+                 {"end_pass": 0, "depth_log": 40, "depth_log_min": 0},
                  {"pull_scale": 0, "pull_pair": 2, "pair_rows": 1, "pair_blocks": 8},
                  {"pull_scale": 0, "pull_pair": 2, "pair_rows": 3, "depth_log": 40, "depth_log_min": 0},
                  {"pull_scale": 0, "pull_pair": 2, "pair_rows": 8, "depth_log": 40, "depth_log_min": 0,
                   "fuse_finalize": 0}]
 PULL_IDS = ["pull_tiles4", "pull_tiles3_chunk5", "pull_tile1_off", "pull_tile1_3", "pull_gate1", "robber_by_new", "robber_by_open", "small_grids", "scan_tiles4", "push_no_marks", "pull_rows", "push_only", "direct_bytes", "pull_log_fused",
             "pull_log_unfused", "depth_log", "small_log", "push_csr", "push_csr_marks", "push_unfiltered",
-            "push_8_blocks", "pull_stage_by_stage", "pair_rows1", "pair_rows3_log", "pair_rows8_log_unfused"]
+            "push_8_blocks", "pull_stage_by_stage", "two_launch_level_end", "two_launch_level_end_log", "pair_rows1", "pair_rows3_log", "pair_rows8_log_unfused"]
 DEFAULTS = {"pull_scale": -1, "pull_tiles": 1, "pull_tile_rows": 4, "pull_tile_chunk": 16, "pull_tile_blocks": 8192, "depth_log": 40, "depth_log_min": 1 << 24,
             "levels_per_sync": 8, "fuse_finalize": 1, "pull_tile1": 9, "pull_gate": 4, "robber_open": 2, "row_blocks": 32768, "log_blocks": 4096, "scan_tile": 8192, "push_dirty": 1,
-            "pred_csr": 0, "push_filter": 1, "push_blocks": 2048, "pull_pair": 1, "pair_rows": 4, "pair_blocks": 1024}
+            "pred_csr": 0, "push_filter": 1, "push_blocks": 2048, "pull_pair": 1, "pair_rows": 4, "pair_blocks": 1024,
+            "end_pass": 1}
 PULL_CASES = ["v1__cycle100__k2__p1", "v1__dodecahedron__k3__p1", "v1__grid36__k3__p1", "v1__grid9__k3__p1",
               "v1__line3__k2__p1", "v1__outerplanar8__k3__p1", "v1__peterson__k3__p1", "v1__planar32__k3__p1",
               "v1__scotlandyard-all__k2__p1", "v1__test__k2__p1"]

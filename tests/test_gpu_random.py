@@ -30,6 +30,7 @@ CHOICES = {
     "pull_pair": [0, 1, 2, 2],
     "pair_rows": [1, 3, 4, 8],
     "pair_blocks": [8, 1024],
+    "end_pass": [0, 1, 1],
 }
 
 
