@@ -31,7 +31,7 @@ STATUS = {
     0: "CR_OK", -1: "CR_ERR_ARG", -2: "CR_ERR_IO", -3: "CR_ERR_EMPTY_GRAPH",
     -4: "CR_ERR_GRAPH_LIMIT", -5: "CR_ERR_ISOLATED_VERTEX", -6: "CR_ERR_K_RANGE",
     -7: "CR_ERR_P_RANGE", -8: "CR_ERR_TOO_LARGE", -9: "CR_ERR_DEVICE", -10: "CR_ERR_ORDER",
-    -11: "CR_ERR_UNSUPPORTED", -12: "CR_ERR_NO_DEVICE",
+    -11: "CR_ERR_UNSUPPORTED", -12: "CR_ERR_NO_DEVICE", -13: "CR_ERR_FORMAT", -14: "CR_ERR_CACHE_RULES",
 }
 
 # Names every build of include/crsolve.h exports.
@@ -48,7 +48,7 @@ EXPORTS = (
     "cr_batch_last_timing", "cr_server_set_tuning",
     "cr_server_create", "cr_server_submit", "cr_server_submit_many", "cr_server_poll", "cr_server_stats", "cr_server_destroy",
     "cr_clock_ns", "cr_batch_solve_noalloc", "cr_server_retain_tables", "cr_server_copy_table",
-    "cr_server_release_table", "cr_server_table_stats",
+    "cr_server_release_table", "cr_server_table_stats", "cr_load_aux_graph", "cr_read_aux_header",
 )
 
 
@@ -132,6 +132,8 @@ def lib():
         L.cr_main_loop.argtypes = [vp, u64p, i32, ctypes.POINTER(i32)]
         L.cr_find_final_result.argtypes = [vp, ctypes.POINTER(Verdict)]
         L.cr_save_aux_graph.argtypes = [vp, ctypes.c_char_p]
+        L.cr_load_aux_graph.argtypes = [vp, ctypes.c_char_p]
+        L.cr_read_aux_header.argtypes = [ctypes.c_char_p, u64p]
         L.cr_get_info.argtypes = [vp, ctypes.POINTER(Info)]
         L.cr_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.cr_copy_states.argtypes = [vp, u8p, ctypes.c_uint64, ctypes.c_uint64]
@@ -263,6 +265,12 @@ class Solver:
     def output_data(self, path):
         _check(lib().cr_save_aux_graph(self._h, str(path).encode()))
 
+    def load_aux_graph(self, path):
+        """CacheManager::loadAuxGraph (CacheManager.h:77-141) in place of
+        build_aux_graph: the blob's states become the table (readable and
+        verdict-ready; initialize_captures resets it)."""
+        _check(lib().cr_load_aux_graph(self._h, str(path).encode()))
+
     # ---- accessors ----
     def info(self):
         i = Info()
@@ -381,6 +389,15 @@ class Solver:
         _check(lib().cr_kernel_times(self._h, names, 4096, ms, la, 32, ctypes.byref(cnt)))
         labels = names.raw.split(b"\0")[:cnt.value]
         return {labels[i].decode(): (float(ms[i]), int(la[i])) for i in range(cnt.value)}
+
+
+def read_aux_header(path):
+    """The 12 header words of a cache blob (CacheManager.h:49-58): k, N,
+    columns, configCount, numStates, item size, cop and robber self-edge
+    flags, and the four section offsets."""
+    h = np.zeros(12, np.uint64)
+    _check(lib().cr_read_aux_header(str(path).encode(), _u64(h)))
+    return h
 
 
 def solve(adj, k, p=1, variant=VARIANT_COLUMNS, device=0):

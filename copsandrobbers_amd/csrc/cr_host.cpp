@@ -2,6 +2,7 @@
 #include "cr_host.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
@@ -223,6 +224,156 @@ int write_aux_blob(const std::string& path, const Adjacency& g, int32_t k, int32
     }
     if (std::fclose(f) != 0) ok = false;
     return ok ? 0 : CR_ERR_IO;
+}
+
+namespace {
+
+struct InFile {
+    std::FILE* f;
+    explicit InFile(const std::string& p) : f(std::fopen(p.c_str(), "rb")) {}
+    ~InFile() {
+        if (f) std::fclose(f);
+    }
+};
+
+bool get(std::FILE* f, void* p, size_t bytes) {
+    return bytes == 0 || std::fread(p, 1, bytes, f) == bytes;
+}
+
+bool seek_to(std::FILE* f, uint64_t off) {
+    return fseeko(f, static_cast<off_t>(off), SEEK_SET) == 0;
+}
+
+}  // namespace
+
+int read_aux_header(const std::string& path, uint64_t header[12], uint64_t* file_size, std::string& why) {
+    InFile in(path);
+    if (!in.f) {
+        why = "cannot open " + path;
+        return CR_ERR_IO;
+    }
+    if (fseeko(in.f, 0, SEEK_END) != 0) {
+        why = "cannot seek in " + path;
+        return CR_ERR_IO;
+    }
+    const off_t end = ftello(in.f);
+    if (end < 0 || !seek_to(in.f, 0)) {
+        why = "cannot seek in " + path;
+        return CR_ERR_IO;
+    }
+    const uint64_t size = static_cast<uint64_t>(end);
+    if (size < 12 * sizeof(uint64_t) || !get(in.f, header, 12 * sizeof(uint64_t))) {
+        why = path + " is shorter than the 96-byte header";
+        return CR_ERR_IO;
+    }
+    // header | configs | transition heads | transitions | states, in this order
+    const uint64_t s1 = header[8], s2 = header[9], s3 = header[10], s4 = header[11];
+    if (s1 != 12 * sizeof(uint64_t) || s2 < s1 || s3 < s2 || s4 < s3 || s4 > size || (s3 - s2) % 8 != 0 ||
+        (s4 - s3) % 8 != 0) {
+        why = "section offsets " + std::to_string(s1) + ", " + std::to_string(s2) + ", " + std::to_string(s3) + ", " +
+              std::to_string(s4) + " do not fit a file of " + std::to_string(size) + " bytes";
+        return CR_ERR_FORMAT;
+    }
+    if (header[5] == 0 || (size - s4) % header[5] != 0 || header[6] > 1 || header[7] > 1) {
+        why = "item size " + std::to_string(header[5]) + " or self-edge flags " + std::to_string(header[6]) +
+              std::to_string(header[7]) + " out of range";
+        return CR_ERR_FORMAT;
+    }
+    if (file_size) *file_size = size;
+    return 0;
+}
+
+int read_aux_blob(const std::string& path, const Adjacency& g, int32_t k, int32_t columns, bool cop_stay,
+                  bool robber_stay, uint64_t num_states, std::string& why,
+                  const std::function<int(uint64_t, uint64_t, const uint8_t*)>& put_states) {
+    uint64_t h[12];
+    uint64_t size = 0;
+    if (int rc = read_aux_header(path, h, &size, why)) return rc;
+    if (h[6] != (cop_stay ? 1u : 0u) || h[7] != (robber_stay ? 1u : 0u)) {
+        why = "Cache rules mismatch: Cached SelfEdges [" + std::to_string(h[6]) + std::to_string(h[7]) +
+              "] vs Requested [" + (cop_stay ? "1" : "0") + (robber_stay ? "1" : "0") + "]";
+        return CR_ERR_CACHE_RULES;
+    }
+    const uint64_t C = config_count(g.n, k);
+    const uint64_t want[6] = {static_cast<uint64_t>(k), static_cast<uint64_t>(g.n), static_cast<uint64_t>(columns),
+                              C, num_states, 1};
+    static const char* const names[6] = {"k", "N", "columns", "configCount", "numStates", "sizeof(DataItem)"};
+    for (int i = 0; i < 6; ++i)
+        if (h[i] != want[i]) {
+            why = std::string("header field ") + names[i] + " is " + std::to_string(h[i]) + "; the loaded graph has " +
+                  std::to_string(want[i]);
+            return CR_ERR_FORMAT;
+        }
+    if (h[9] != h[8] + C * static_cast<uint64_t>(k) || h[10] != h[9] + (C + 1) * sizeof(uint64_t) ||
+        size - h[11] != num_states) {
+        why = "section sizes do not match " + std::to_string(C) + " configurations and " + std::to_string(num_states) +
+              " states";
+        return CR_ERR_FORMAT;
+    }
+    InFile in(path);
+    if (!in.f || !seek_to(in.f, h[8])) {
+        why = "cannot read " + path;
+        return CR_ERR_IO;
+    }
+    {  // configs: the ranking's order (AuxGraph.h:155-176), in chunks
+        const uint64_t chunk = 1 << 20;
+        std::vector<uint8_t> buf;
+        for (uint64_t c0 = 0; c0 < C; c0 += chunk) {
+            const uint64_t c1 = std::min(C, c0 + chunk);
+            buf.resize((c1 - c0) * k);
+            if (!get(in.f, buf.data(), buf.size())) {
+                why = "cannot read the configs section of " + path;
+                return CR_ERR_IO;
+            }
+            std::atomic<uint64_t> bad{~0ull};
+            parallel_ranges(c1 - c0, [&](uint64_t lo, uint64_t hi) {
+                uint8_t t[CR_MAX_K];
+                for (uint64_t c = lo; c < hi; ++c) {
+                    unrank_tuple(c0 + c, g.n, k, t);
+                    if (std::memcmp(t, &buf[c * k], k) != 0) {
+                        uint64_t cur = bad.load();
+                        while (c0 + c < cur && !bad.compare_exchange_weak(cur, c0 + c)) {
+                        }
+                        return;
+                    }
+                }
+            });
+            if (bad.load() != ~0ull) {
+                why = "configuration " + std::to_string(bad.load()) + " differs from the ranking's";
+                return CR_ERR_FORMAT;
+            }
+        }
+    }
+    uint64_t head0 = 1, headC = 0;
+    if (!get(in.f, &head0, sizeof head0) || !seek_to(in.f, h[9] + C * sizeof(uint64_t)) ||
+        !get(in.f, &headC, sizeof headC)) {
+        why = "cannot read the transition heads of " + path;
+        return CR_ERR_IO;
+    }
+    if (head0 != 0 || headC != (h[11] - h[10]) / sizeof(uint64_t)) {
+        why = "transition heads (" + std::to_string(head0) + " .. " + std::to_string(headC) +
+              ") do not span the transitions section";
+        return CR_ERR_FORMAT;
+    }
+    if (!seek_to(in.f, h[11])) {
+        why = "cannot seek to the states of " + path;
+        return CR_ERR_IO;
+    }
+    const uint64_t chunk = 64ull << 20;
+    std::vector<uint8_t> buf;
+    for (uint64_t o = 0; o < num_states; o += chunk) {
+        const uint64_t cnt = std::min(chunk, num_states - o);
+        buf.resize(cnt);
+        if (!get(in.f, buf.data(), cnt)) {
+            why = "cannot read the states section of " + path;
+            return CR_ERR_IO;
+        }
+        if (int rc = put_states(o, cnt, buf.data())) {
+            why = "could not store the states";
+            return rc;
+        }
+    }
+    return 0;
 }
 
 int32_t parse_edge_list(const uint8_t* text, uint64_t len, uint8_t* adj, int32_t cap) {

@@ -48,6 +48,21 @@ int write_aux_blob(const std::string& path, const Adjacency& g, int32_t k, int32
                    bool cop_stay, bool robber_stay, uint64_t num_states,
                    const std::function<int(uint64_t, uint64_t, uint8_t*)>& fetch_states);
 
+// The 96-byte header of a cache blob (CacheManager.h:49-58) and the file's
+// size, with the section offsets checked against each other and the size.
+// Returns 0, CR_ERR_IO or CR_ERR_FORMAT (why says what failed).
+int read_aux_header(const std::string& path, uint64_t header[12], uint64_t* file_size, std::string& why);
+
+// CacheManager::loadAuxGraph, CacheManager.h:77-141, for a graph already
+// loaded: the rules flags first (CR_ERR_CACHE_RULES, :89-95), then every
+// header field against what write_aux_blob writes for (g, k, columns, rules,
+// num_states), the configs section against the ranking, heads[0] and heads[C]
+// against the transitions section (CR_ERR_FORMAT). The states section goes to
+// put_states(offset, count, src) in chunks; the transitions are not read.
+int read_aux_blob(const std::string& path, const Adjacency& g, int32_t k, int32_t columns, bool cop_stay,
+                  bool robber_stay, uint64_t num_states, std::string& why,
+                  const std::function<int(uint64_t, uint64_t, const uint8_t*)>& put_states);
+
 // legacy/util/edge_parser.py:4-63: 1-based "u v" / "u, v" lines, '#'
 // comments, undirected; N = the largest id (an id 0 lands on row N-1, as
 // Python's negative index does there). Returns N, 0 when there is no edge.

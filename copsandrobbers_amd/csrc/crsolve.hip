@@ -1962,6 +1962,9 @@ __global__ __launch_bounds__(kBlock) void k_end_level(LevelArgs a, uint32_t d) {
         }
         const uint32_t pos = s_base + s_gpre[threadIdx.x / kGroupWords] + incl - pc;
         write_word_entries(bits, pc, pos, word, sp ? a.list1[nxt] : a.list0[nxt], sp ? a.list0[cur] : nullptr, small);
+        // a grid with a workgroup per tile takes one ticket each (the tickets
+        // are same-address atomics, which serialise)
+        if (gridDim.x >= T0 + T1) break;  // uniform
         __syncthreads();  // s_tile, s_base and s_gpre are rewritten by the next tile
     }
 }
@@ -2286,6 +2289,22 @@ __global__ __launch_bounds__(kBlock) void k_verdict(GraphView gv, LevelArgs a, i
     if (lane == 0 && best != ~0ull) atomicMin(&a.ctl->verdict, best);
 }
 
+// The won marks of column 0 (the cop-turn column) of a table read from a
+// cache blob, as the verdict's universal-row test reads them: lanes = rows.
+__global__ __launch_bounds__(kBlock) void k_marks_from_table(const uint8_t* table, uint64_t C, int32_t n,
+                                                             int32_t columns, int32_t W, uint64_t* marks) {
+    for (uint64_t row = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; row < C;
+         row += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint8_t* t = table + row * static_cast<uint64_t>(n) * columns;
+        for (int q = 0; q < W; ++q) {
+            uint64_t m = 0;
+            for (int r = 64 * q; r < min(n, 64 * q + 64); ++r)
+                m |= static_cast<uint64_t>(t[static_cast<uint64_t>(r) * columns] & 1u) << (r & 63);
+            marks[row * W + q] = m;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // retro-edge count (metric only): |pred(C)| by an LDS hash set per wave
 // ---------------------------------------------------------------------------
@@ -2477,6 +2496,8 @@ struct cr_solver {
     uint32_t* d_tile_cnt = nullptr;
     uint8_t* d_dirty = nullptr;               // per group: marked by the push for k_cop_finalize
     unsigned long long* d_lb_status = nullptr;  // k_end_level: published tile sums, [2][lb_stride]
+    bool from_blob = false;                   // the table was read by cr_load_aux_graph (no solve state)
+    uint64_t* d_marks = nullptr;              // won marks of column 0 of a table read from a blob
     uint32_t lb_stride = 0;
     bool end_pass = true;                     // tuning: one k_end_level instead of k_advance + k_write_lists
     bool push_dirty = true;
@@ -2553,6 +2574,9 @@ struct cr_solver {
         d_dirty = nullptr;
         hipFree(d_lb_status);
         d_lb_status = nullptr;
+        hipFree(d_marks);
+        d_marks = nullptr;
+        from_blob = false;
         if (d_pair) hipFree(d_pair);
         d_pair = nullptr;
         pair_cap = 0;
@@ -3501,6 +3525,7 @@ int init_macro(cr_solver* s, uint64_t* base) {
 
 int do_init(cr_solver* s, uint64_t* base) {
     if (s->stage < kBuilt) return fail(CR_ERR_ORDER, "initializeCaptures before buildAuxGraph");
+    s->from_blob = false;
     int rc = reset_ctl(s);
     if (rc) return rc;
     if (s->variant == CR_VARIANT_MACRO) return init_macro(s, base);
@@ -3564,7 +3589,7 @@ int loop_macro(cr_solver* s) {
 }
 
 int do_loop(cr_solver* s) {
-    if (s->stage < kInitialized) return fail(CR_ERR_ORDER, "mainLoop before initializeCaptures");
+    if (s->stage < kInitialized || s->from_blob) return fail(CR_ERR_ORDER, "mainLoop before initializeCaptures");
     s->ktimes.clear();
     s->korder.clear();
     if (s->variant == CR_VARIANT_MACRO) return loop_macro(s);
@@ -3889,7 +3914,10 @@ int cr_find_final_result(cr_solver* s, cr_verdict* out) {
     const unsigned long long init = ~0ull;
     HIP_TRY(hipMemcpyAsync(&s->d_ctl->verdict, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
     LevelArgs a = level_args(s);
-    if (s->colplan) a.won0 = crd::columns_won0(s->colplan);  // column 0 of the p >= 2 solver
+    if (s->from_blob)
+        a.won0 = s->d_marks;  // derived from the table read from the blob
+    else if (s->colplan)
+        a.won0 = crd::columns_won0(s->colplan);  // column 0 of the p >= 2 solver
     if (int rc = dispatch_w<LaunchVerdict>(s->W, s, a)) return rc;
     HIP_TRY(hipGetLastError());
     unsigned long long v = 0;
@@ -3920,6 +3948,44 @@ int cr_save_aux_graph(cr_solver* s, const char* path) {
     };
     const int rc = cr::write_aux_blob(path, s->g, s->k, s->columns, s->cop_stay != 0, s->robber_stay != 0, s->S, fetch);
     if (rc != 0) return fail(rc, "could not write %s", path);
+    return 0;
+}
+
+int cr_load_aux_graph(cr_solver* s, const char* path) {
+    if (int rc = check_solver(s)) return rc;
+    if (path == nullptr) return fail(CR_ERR_ARG, "null path");
+    if (s->stage < kLoaded) return fail(CR_ERR_ORDER, "loadAuxGraph before loadGraphFile");
+    HIP_TRY(hipSetDevice(s->device));
+    uint64_t header[12];
+    std::string why;
+    if (int rc = cr::read_aux_header(path, header, nullptr, why)) return fail(rc, "%s", why.c_str());
+    if (header[6] != (s->cop_stay ? 1u : 0u) || header[7] != (s->robber_stay ? 1u : 0u))  // before any allocation
+        return fail(CR_ERR_CACHE_RULES, "Cache rules mismatch: Cached SelfEdges [%llu%llu] vs Requested [%d%d]",
+                    static_cast<unsigned long long>(header[6]), static_cast<unsigned long long>(header[7]),
+                    s->cop_stay ? 1 : 0, s->robber_stay ? 1 : 0);
+    if (int rc = do_build(s)) return rc;
+    auto put = [s](uint64_t off, uint64_t cnt, const uint8_t* src) -> int {
+        if (hipMemcpyAsync(s->d_table + off, src, cnt, hipMemcpyHostToDevice, s->stream) != hipSuccess) return CR_ERR_DEVICE;
+        return hipStreamSynchronize(s->stream) == hipSuccess ? 0 : CR_ERR_DEVICE;
+    };
+    if (int rc = cr::read_aux_blob(path, s->g, s->k, s->columns, s->cop_stay != 0, s->robber_stay != 0, s->S, why, put))
+        return fail(rc, "%s: %s", path, why.c_str());
+    int rc = 0;
+    if (!(s->d_marks = dev_alloc<uint64_t>(s->C * s->W, s->device_bytes, rc))) return rc;
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((s->C + kBlock - 1) / kBlock, 4096));
+    hipLaunchKernelGGL(k_marks_from_table, dim3(std::max<uint32_t>(grid, 1)), dim3(kBlock), 0, s->stream, s->d_table,
+                       static_cast<uint64_t>(s->C), s->g.n, s->columns, static_cast<int32_t>(s->W), s->d_marks);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->from_blob = true;
+    s->stage = kSolved;
+    return 0;
+}
+
+int cr_read_aux_header(const char* path, uint64_t* header12) {
+    if (path == nullptr || header12 == nullptr) return fail(CR_ERR_ARG, "null argument");
+    std::string why;
+    if (int rc = cr::read_aux_header(path, header12, nullptr, why)) return fail(rc, "%s", why.c_str());
     return 0;
 }
 
@@ -3971,6 +4037,7 @@ int cr_level_counts(cr_solver* s, uint64_t* out) {
         return fail(CR_ERR_UNSUPPORTED, "depth levels and retro-edges are kept for the p = 1 column solver");
     if (out == nullptr) return fail(CR_ERR_ARG, "null output");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    if (s->from_blob) return fail(CR_ERR_ORDER, "a table read from a blob has no level data: solve it first");
     for (int d = 0; d < 127; ++d) {
         out[2 * d] = s->h_ctl->lvl[d][0];
         out[2 * d + 1] = s->h_ctl->lvl[d][1];
@@ -3985,6 +4052,7 @@ int cr_count_retro_edges(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint64_
     if (s->variant == CR_VARIANT_MACRO || s->p != 1)
         return fail(CR_ERR_UNSUPPORTED, "depth levels and retro-edges are kept for the p = 1 column solver");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    if (s->from_blob) return fail(CR_ERR_ORDER, "a table read from a blob has no level data: solve it first");
     unsigned long long r[3];
     if (int rc = count_moves(s, 0, r)) return rc;
     if (e_cop) *e_cop = r[0];
@@ -4000,6 +4068,7 @@ int cr_work_profile(cr_solver* s, uint64_t* e_cop, uint64_t* e_rob, uint8_t* pul
     if (s->variant == CR_VARIANT_MACRO || s->p != 1)
         return fail(CR_ERR_UNSUPPORTED, "depth levels and retro-edges are kept for the p = 1 column solver");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    if (s->from_blob) return fail(CR_ERR_ORDER, "a table read from a blob has no level data: solve it first");
     unsigned long long r[3];
     if (int rc = count_moves(s, 0, r)) return rc;
     Ctl* h = s->h_ctl;
@@ -4019,6 +4088,7 @@ int cr_level_rows(cr_solver* s, uint64_t* rows) {
     if (s->variant == CR_VARIANT_MACRO || s->p != 1)
         return fail(CR_ERR_UNSUPPORTED, "depth levels are kept for the p = 1 column solver");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    if (s->from_blob) return fail(CR_ERR_ORDER, "a table read from a blob has no level data: solve it first");
     for (int d = 0; d < 127; ++d) {
         rows[2 * d] = s->h_ctl->rows[d][0];
         rows[2 * d + 1] = s->h_ctl->rows[d][1];
@@ -4032,6 +4102,7 @@ int cr_get_macro_stats(cr_solver* s, cr_macro_stats* out, uint64_t* nodes_per_st
     if (out == nullptr) return fail(CR_ERR_ARG, "null output");
     if (s->variant != CR_VARIANT_MACRO) return fail(CR_ERR_UNSUPPORTED, "no trajectory DAG in the column solver");
     if (s->stage < kSolved) return fail(CR_ERR_ORDER, "not solved");
+    if (s->from_blob) return fail(CR_ERR_ORDER, "a table read from a blob has no level data: solve it first");
     const crd::MacroProfile& m = s->mprof;
     std::memset(out, 0, sizeof *out);
     out->steps = static_cast<int32_t>(m.nodes.size());

@@ -50,7 +50,9 @@ enum cr_status {
     CR_ERR_DEVICE = -9,          /* HIP runtime error */
     CR_ERR_ORDER = -10,          /* step called out of order */
     CR_ERR_UNSUPPORTED = -11,    /* rule combination not implemented on the device */
-    CR_ERR_NO_DEVICE = -12       /* no HIP device visible */
+    CR_ERR_NO_DEVICE = -12,      /* no HIP device visible */
+    CR_ERR_FORMAT = -13,         /* a cache blob whose layout does not fit the loaded graph */
+    CR_ERR_CACHE_RULES = -14     /* a cache blob solved under other self-edge rules (CacheManager.h:89-95) */
 };
 
 /* Solver variants: which reference solver translation unit is replaced. */
@@ -142,6 +144,23 @@ int cr_find_final_result(cr_solver* s, cr_verdict* out);
  * 96-byte header, configs, transition heads, transitions (nextId*N) and
  * states to path. */
 int cr_save_aux_graph(cr_solver* s, const char* path);
+/* CacheManager::loadAuxGraph, CacheManager.h:77-141, in place of buildAuxGraph
+ * (k_cops_visibility.cpp:66-71): reads the blob at path for the graph, k, p,
+ * variant and rules loaded in s. The blob's self-edge flags must equal the
+ * solver's rules (CR_ERR_CACHE_RULES, with the reference's "Cache rules
+ * mismatch" text in cr_last_error). Every other header field, the section
+ * offsets, the file size, the configs section and the ends of the transition
+ * heads must be what cr_save_aux_graph writes for this graph (CR_ERR_FORMAT;
+ * the reference takes them from the header, here the device layout is fixed
+ * by the graph). The states section becomes the device table: cr_copy_states,
+ * cr_find_final_result, cr_write_dp_table, cr_extract_game and
+ * cr_save_aux_graph read it as after a solve; the level and retro-edge
+ * accessors and cr_main_loop need cr_initialize_captures first, which resets
+ * the table as the reference's does on a cache hit (k_cops_visibility.cpp:90-104). */
+int cr_load_aux_graph(cr_solver* s, const char* path);
+/* The 96-byte header of a cache blob (CacheManager.h:49-58), after checking
+ * the section offsets against each other and the file size (no device). */
+int cr_read_aux_header(const char* path, uint64_t* header12);
 
 /* ---- accessors ------------------------------------------------------- */
 int cr_get_info(const cr_solver* s, cr_info* out);
