@@ -420,23 +420,32 @@ __global__ __launch_bounds__(kBlock) void k_cop_push(GraphView gv, LevelArgs a) 
                 keep[u] = product_tuple<K>(in ? tt : t, cops, deg, base, in_nbr, tup[u]) && in;
                 pr[u] = rank_sorted<K>(tup[u], rank_tab, gv.n, gv.ctot_m1);
             }
-            for (int w = 0; w < W; ++w) {
-                if (m[w] == 0) continue;
-                uint64_t mw[2];
-#pragma unroll
-                for (int u = 0; u < 2; ++u) mw[u] = keep[u] ? m[w] : 0ull;
-                if (a.filter) {
-#pragma unroll
-                    for (int u = 0; u < 2; ++u)
-                        if (mw[u]) mw[u] &= ~a.won0[pr[u] * W + w];
-                }
+            // every filter load of the iteration before any atomic: the
+            // atomics may alias won0 as far as the compiler knows, so loads
+            // interleaved with them would wait one after another
+            uint64_t wv[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+            if (a.filter) {
 #pragma unroll
                 for (int u = 0; u < 2; ++u)
-                    if (mw[u]) {
+                    if (keep[u]) {
+                        const uint64_t* src = a.won0 + pr[u] * W;
+#pragma unroll
+                        for (int w = 0; w < 4; ++w)
+                            if (w < W && m[w]) wv[u][w] = src[w];
+                    }
+            }
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                if (w >= W || m[w] == 0) continue;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const uint64_t mw = keep[u] ? m[w] & ~wv[u][w] : 0ull;
+                    if (mw) {
                         atomicOr(reinterpret_cast<unsigned long long*>(a.nx0 + pr[u] * W + w),
-                                 static_cast<unsigned long long>(mw[u]));
+                                 static_cast<unsigned long long>(mw));
                         hit[u] = true;
                     }
+                }
             }
             if (a.dirty) {  // k_cop_finalize reads only the groups marked here
 #pragma unroll
@@ -2201,23 +2210,30 @@ __global__ __launch_bounds__(kBlock) void k_cop_push_csr(GraphView gv, LevelArgs
                 keep[u] = i + 64 * u < end;
                 pr[u] = keep[u] ? csr[i + 64 * u] : 0u;
             }
-            for (int w = 0; w < W; ++w) {
-                if (m[w] == 0) continue;
-                uint64_t mw[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) mw[u] = keep[u] ? m[w] : 0ull;
-                if (a.filter) {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (mw[u]) mw[u] &= ~a.won0[pr[u] * W + w];
-                }
+            // filter loads first, then the atomics (as in k_cop_push)
+            uint64_t wv[4][4] = {};
+            if (a.filter) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    if (mw[u]) {
+                    if (keep[u]) {
+                        const uint64_t* src = a.won0 + pr[u] * W;
+#pragma unroll
+                        for (int w = 0; w < 4; ++w)
+                            if (w < W && m[w]) wv[u][w] = src[w];
+                    }
+            }
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                if (w >= W || m[w] == 0) continue;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint64_t mw = keep[u] ? m[w] & ~wv[u][w] : 0ull;
+                    if (mw) {
                         atomicOr(reinterpret_cast<unsigned long long*>(a.nx0 + pr[u] * W + w),
-                                 static_cast<unsigned long long>(mw[u]));
+                                 static_cast<unsigned long long>(mw));
                         hit[u] = true;
                     }
+                }
             }
             if (a.dirty) {
 #pragma unroll
