@@ -1307,14 +1307,20 @@ __global__ __launch_bounds__(kBlock) void k_clear_frontier(LevelArgs a, int32_t 
 // nibbles of the open set: lds_nib[(p * 16 + x) * W + q] = word q of the OR of
 // the in-masks of the vertices 4p + j for the set bits j of x. A row then
 // costs one table row per nibble (n / 4) whatever its density.
-constexpr int kNibbles = (CR_MAX_N + 3) / 4;
 constexpr uint32_t kNibbleMin = 24;  // open states above which a row takes the nibble table
+// 512-thread workgroups: two groups of rows share one copy of the masks and the
+// nibble table, which bound the step's occupancy
+constexpr int kRobberBlock = 2 * kBlock;
+constexpr int kRobberGroups = kRobberBlock / 64 / kWavesPerBlock;
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs a) {
-    __shared__ uint64_t lds_om[CR_MAX_N * W];
-    __shared__ uint64_t lds_im[CR_MAX_N * W];
-    __shared__ uint64_t lds_nib[kNibbles * 16 * W];
+__global__ __launch_bounds__(kRobberBlock, 8) void k_robber_step(GraphView gv, LevelArgs a) {
+    // out-masks, in-masks and the nibble table, sized for this graph (the
+    // occupancy of the step is bound by this LDS; LaunchRobber sizes it)
+    extern __shared__ __align__(16) uint64_t lds_rob[];
+    uint64_t* lds_om = lds_rob;
+    uint64_t* lds_im = lds_rob + gv.n * W;
+    uint64_t* lds_nib = lds_rob + 2 * gv.n * W;
     const int lane = threadIdx.x & 63;
     if (a.ctl->done) return;
     const uint32_t level = a.ctl->level, cur = level & 1;
@@ -1322,7 +1328,7 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
     const uint32_t* list = a.list0[cur];
     const uint32_t nwords = (count + 63) / 64;
     const uint32_t ngroups = (nwords + kGroupWords - 1) / kGroupWords;
-    if (blockIdx.x >= ngroups) return;  // no rows here: skip the mask staging (uniform)
+    if (blockIdx.x * kRobberGroups >= ngroups) return;  // no rows here: skip the mask staging (uniform)
     const uint64_t* lom = stage_out_masks<W>(lds_om, a.out_mask, gv.n);
     const uint64_t* lim = stage_out_masks<W>(lds_im, a.in_mask, gv.n);
     const int nnib = (gv.n + 3) / 4;
@@ -1340,7 +1346,9 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
         for (int q = 0; q < W; ++q) lds_nib[e * W + q] = acc[q];
     }
     __syncthreads();
-    const uint32_t wid = wave_id();
+    // kRobberGroups groups per pass, kWavesPerBlock waves (one flag word each) per group
+    const uint32_t wid = wave_id() % kWavesPerBlock, half = wave_id() / kWavesPerBlock;
+    __shared__ unsigned long long s_red[2][kRobberBlock / 64];
     const int n = gv.n;
     const uint8_t nb = won_byte(level + 1);
     uint64_t full[W];  // the row's robber positions
@@ -1349,7 +1357,8 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
         const int bits = min(64, max(0, n - 64 * q));
         full[q] = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
     }
-    for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    for (uint32_t g0 = blockIdx.x * kRobberGroups; g0 < ngroups; g0 += gridDim.x * kRobberGroups) {
+        const uint32_t g = g0 + half;  // past the end for the second half of the last pass: no word
         unsigned long long rows = 0, found = 0;
         for (int j = 0; j < kWordsPerWave; ++j) {
             const uint32_t b = g * kGroupWords + wid * kWordsPerWave + j;
@@ -1480,8 +1489,18 @@ __global__ __launch_bounds__(kBlock) void k_robber_step(GraphView gv, LevelArgs 
             rows += __popcll(f);
             found += wave_sum(pop);
         }
-        block_sum2(rows, found);
-        if (threadIdx.x == 0) {
+        __syncthreads();  // s_red of the previous pass is read
+        if (lane == 0) {
+            s_red[0][wave_id()] = rows;
+            s_red[1][wave_id()] = found;
+        }
+        __syncthreads();
+        if (wid == 0 && lane == 0 && g < ngroups) {
+            rows = found = 0;
+            for (int i = 0; i < kWavesPerBlock; ++i) {
+                rows += s_red[0][half * kWavesPerBlock + i];
+                found += s_red[1][half * kWavesPerBlock + i];
+            }
             a.c1.gcnt[g] = static_cast<uint32_t>(rows);
             a.c1.gstates[g] = found;
         }
@@ -2925,7 +2944,9 @@ struct LaunchPull {
 template <int W>
 struct LaunchRobber {
     static int run(cr_solver* s, LevelArgs a) {
-        hipLaunchKernelGGL(k_robber_step<W>, dim3(s->row_grid()), dim3(kBlock), 0, s->stream, s->gv, a);
+        const uint32_t lds = (2 * s->g.n + 16 * ((s->g.n + 3) / 4)) * W * 8;  // k_robber_step's LDS
+        const uint32_t grid = (s->row_grid() + kRobberGroups - 1) / kRobberGroups;
+        hipLaunchKernelGGL(k_robber_step<W>, dim3(grid), dim3(kRobberBlock), lds, s->stream, s->gv, a);
         return 0;
     }
 };
