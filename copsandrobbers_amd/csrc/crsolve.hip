@@ -87,6 +87,7 @@ struct Ctl {
     uint32_t log_open;              // the lists of the depth that ended the loop are still to be written
     unsigned long long log_used;    // entries in use
     unsigned long long log_cap;     // capacity in entries (0: no log)
+    uint32_t log_first;             // the first depth of the log since the last application
     unsigned long long log_off[128][2];  // per depth and turn: first entry
     uint32_t log_len[128][2];            // per depth and turn: entries (0 when written directly)
     unsigned long long lvl[128][2];
@@ -142,6 +143,9 @@ struct LevelArgs {
     uint32_t pull_rows;        // a level pulls when its robber frontier has more rows than this
     uint32_t* log_rows;        // depth log: configuration of each entry
     uint64_t* log_bits;        // depth log: W robber words of each entry
+    uint32_t* log_beg;         // depth log: per list and chunk of log_R rows, the first and one past the
+    uint32_t* log_end;         //   last entry with rows in the chunk (end 0: none), [kMaxLogLists][log_chunks]
+    uint32_t log_R, log_chunks;
     int32_t n, W;
     int32_t fuse;              // pulled levels: the last pull stage does k_cop_finalize's work
     uint32_t open_factor;      // robber step: rows with open <= open_factor * new cop-turn states take the open-state path
@@ -2024,6 +2028,13 @@ __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
             const unsigned long long e = (robber ? off1 : off0) + k;
             a.log_rows[e] = row;
             store_row<W>(a.log_bits + e * W, w);
+            // the list's range in each chunk: its first entry there writes the
+            // begin, its last the end (the list is sorted by row)
+            const uint32_t* list = robber ? a.list1[nxt] : a.list0[nxt];
+            const uint32_t len = robber ? n1 : n0, c = row / a.log_R;
+            const uint64_t cell = static_cast<uint64_t>(2 * (depth - ctl->log_first) + (robber ? 1 : 0)) * a.log_chunks + c;
+            if (k == 0 || list[k - 1] / a.log_R != c) a.log_beg[cell] = k;
+            if (k + 1 == len || list[k + 1] / a.log_R != c) a.log_end[cell] = k + 1;
         } else {
             uint8_t* trow = a.table + static_cast<uint64_t>(row) * a.n * 2 + (robber ? 1 : 0);
 #pragma unroll
@@ -2044,43 +2055,15 @@ __global__ __launch_bounds__(kBlock) void k_log_lists(LevelArgs a) {
 // into the table would touch a few bytes of many 32-byte sectors of a row at
 // every depth. Instead the table is taken in chunks of R rows through LDS
 // (coalesced), every logged depth's entries for the chunk are applied there,
-// and the chunk is written back once. k_log_starts first finds, for every
-// chunk and list, the list's first entry in the chunk.
+// and the chunk is written back once. k_log_lists has recorded, for every
+// list and chunk, the list's entries with rows in the chunk.
 // ---------------------------------------------------------------------------
+constexpr int kMaxLogLists = 256;  // lists (depth, turn) of one log segment
 struct LogList {
     unsigned long long off;    // first entry in the log
-    unsigned long long first;  // entries of the preceding lists (flat index of entry 0)
     uint32_t len;              // entries
     uint32_t turn_byte;        // turn << 8 | depth byte
 };
-
-// Per chunk c and list L: the entries of L with rows in chunk c are
-// [beg, end) at index L * nchunks + c (end is zeroed before; beg is only read
-// where end > 0). One thread per entry; the lists are sorted by row, so the
-// first entry of a chunk writes its beg and the last its end (the
-// neighbouring entries' chunks decide).
-__global__ __launch_bounds__(kBlock) void k_log_chunks(const LogList* lists, int32_t nl, uint64_t total,
-                                                       const uint32_t* rows, uint32_t* beg, uint32_t* end,
-                                                       uint32_t R, uint32_t nchunks) {
-    __shared__ LogList sl[256];
-    for (int L = threadIdx.x; L < nl; L += blockDim.x) sl[L] = lists[L];
-    __syncthreads();
-    for (uint64_t g = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; g < total;
-         g += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        int lo = 0, hi = nl - 1;  // the list holding flat entry g
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (sl[mid].first <= g) lo = mid; else hi = mid - 1;
-        }
-        const LogList ll = sl[lo];
-        const uint32_t i = static_cast<uint32_t>(g - ll.first);
-        const uint32_t* lr = rows + ll.off;
-        const uint32_t c = lr[i] / R;
-        // list-major: consecutive chunks of a list store side by side
-        if (i == 0 || lr[i - 1] / R != c) beg[static_cast<uint64_t>(lo) * nchunks + c] = i;
-        if (i + 1 == ll.len || lr[i + 1] / R != c) end[static_cast<uint64_t>(lo) * nchunks + c] = i + 1;
-    }
-}
 
 __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
@@ -2496,7 +2479,7 @@ struct cr_solver {
     uint32_t log_factor = 40;      // tuning: log capacity in units of C entries (0: no log)
     uint64_t log_min_configs = 1ull << 24;  // tuning: no log for fewer configurations
     LogList* d_lists = nullptr;    // descriptors of the depths k_apply_log applies
-    uint32_t* d_starts = nullptr;  // k_log_chunks: per chunk and list, first entry and count
+    uint32_t* d_starts = nullptr;  // depth log: per list and chunk, first entry and one past the last (k_log_lists)
     LogList* h_lists = nullptr;    // pinned staging of the descriptors
     uint32_t log_first = 0;        // first depth whose lists are not in the table yet
     uint64_t* d_flags[2] = {nullptr, nullptr};                          // producer flag words per turn
@@ -2983,6 +2966,8 @@ struct LaunchRetro {
     }
 };
 
+uint32_t log_chunk_rows(const cr_solver* s);
+
 LevelArgs level_args(cr_solver* s) {
     LevelArgs a{};
     a.won0 = s->d_won0;
@@ -2993,6 +2978,12 @@ LevelArgs level_args(cr_solver* s) {
     a.table = s->d_table;
     a.log_rows = s->d_log_rows;
     a.log_bits = s->d_log_bits;
+    if (s->d_starts) {
+        a.log_R = log_chunk_rows(s);
+        a.log_chunks = static_cast<uint32_t>((s->C + a.log_R - 1) / a.log_R);
+        a.log_beg = s->d_starts;
+        a.log_end = s->d_starts + static_cast<uint64_t>(a.log_chunks) * kMaxLogLists;
+    }
     a.n = s->g.n;
     a.W = s->W;
     // fused finalize: the depth log, LDS tiles, and two robber-frontier
@@ -3229,8 +3220,6 @@ int prepare_pair(cr_solver* s) {
 // and none for fewer than log_min_configs configurations: small tables take
 // their few bytes per depth directly). A depth whose lists do not fit in the
 // remaining room writes its bytes directly (k_log_lists).
-constexpr int kMaxLogLists = 256;
-uint32_t log_chunk_rows(const cr_solver* s);
 
 int alloc_log(cr_solver* s) {
     if (!s->log_factor || s->C < s->log_min_configs) return 0;
@@ -3270,15 +3259,13 @@ struct LaunchApplyLog {
         const uint64_t cells = static_cast<uint64_t>(nchunks) * nl;
         uint32_t* beg = s->d_starts;
         uint32_t* end = s->d_starts + static_cast<uint64_t>(nchunks) * kMaxLogLists;
-        hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<uint32_t>(std::min<uint64_t>((cells + kBlock - 1) / kBlock, 4096))),
-                           dim3(kBlock), 0, s->stream, end, cells, 0u);
-        const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((total + kBlock - 1) / kBlock, 8192)));
-        hipLaunchKernelGGL(k_log_chunks, dim3(sgrid), dim3(kBlock), 0, s->stream, s->d_lists, nl, total, s->d_log_rows,
-                           beg, end, R, nchunks);
         const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nchunks, 65536));
         if (int rc = raise_lds_limit(reinterpret_cast<const void*>(k_apply_log<W>), lds)) return rc;
         hipLaunchKernelGGL(k_apply_log<W>, dim3(grid), dim3(kBlock), lds, s->stream, s->d_lists, nl, s->d_log_rows,
                            s->d_log_bits, beg, end, s->d_table, n, static_cast<uint32_t>(s->C), R);
+        // cleared for the lists of the next segment (k_log_lists sets only chunks with entries)
+        hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<uint32_t>(std::min<uint64_t>((cells + kBlock - 1) / kBlock, 4096))),
+                           dim3(kBlock), 0, s->stream, end, cells, 0u);
         return 0;
     }
 };
@@ -3302,24 +3289,29 @@ int flush_log(cr_solver* s, bool force) {
     for (uint32_t d = s->log_first; d <= last; ++d)
         peak = std::max<uint64_t>(peak, static_cast<uint64_t>(h->log_len[d][0]) + h->log_len[d][1]);
     if (!force && h->log_used + std::max<uint64_t>(peak, 1) * s->levels_per_sync <= s->log_cap) return 0;
-    int32_t nl = 0;
+    // list L = 2 (d - log_first) + turn, as k_log_lists indexed its chunk ranges
+    // (lists written straight to the table are empty here)
+    const int32_t nl = static_cast<int32_t>(2 * (last - s->log_first + 1));
+    if (nl > kMaxLogLists) return fail(CR_ERR_DEVICE, "depth log: %d lists", nl);
     uint64_t total = 0;
     for (uint32_t d = s->log_first; d <= last; ++d)
-        for (int t = 0; t < 2; ++t)
-            if (h->log_len[d][t] && nl < kMaxLogLists) {
-                s->h_lists[nl++] = LogList{h->log_off[d][t], total, h->log_len[d][t],
-                                           static_cast<uint32_t>(t << 8) | static_cast<uint32_t>(1u | (d << 1))};
-                total += h->log_len[d][t];
-            }
-    if (nl) {
+        for (int t = 0; t < 2; ++t) {
+            s->h_lists[2 * (d - s->log_first) + t] =
+                LogList{h->log_off[d][t], h->log_len[d][t], static_cast<uint32_t>(t << 8) | static_cast<uint32_t>(1u | (d << 1))};
+            total += h->log_len[d][t];
+        }
+    if (total) {
         HIP_TRY(hipMemcpyAsync(s->d_lists, s->h_lists, nl * sizeof(LogList), hipMemcpyHostToDevice, s->stream));
         Timer* t = start_timer(s, "apply_log");
         if (int rc = dispatch_w<LaunchApplyLog>(s->W, s, nl, total)) return rc;
         t->stop();
     }
     static const unsigned long long zero = 0;
-    HIP_TRY(hipMemcpyAsync(&s->d_ctl->log_used, &zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
     s->log_first = last + 1;
+    HIP_TRY(hipMemcpyAsync(&s->d_ctl->log_used, &zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(&s->d_ctl->log_first, &s->log_first, sizeof s->log_first, hipMemcpyHostToDevice, s->stream));
+    // the next segment's lists start from cleared chunk ends
+    HIP_TRY(hipStreamSynchronize(s->stream));  // the copies read host memory that changes after return
     return 0;
 }
 
@@ -3470,6 +3462,12 @@ int reset_ctl(cr_solver* s) {
     HIP_TRY(hipMemcpyAsync(s->d_ctl, s->h_ctl, sizeof(Ctl), hipMemcpyHostToDevice, s->stream));
     if (s->d_lb_status)  // tile sums of the last solve carry the same level epochs
         HIP_TRY(hipMemsetAsync(s->d_lb_status, 0, 2ull * s->lb_stride * sizeof(unsigned long long), s->stream));
+    if (s->d_starts) {  // chunk ends of the depth log's lists (a solve that stopped early may leave some)
+        const LevelArgs a = level_args(s);
+        const uint64_t cells = static_cast<uint64_t>(a.log_chunks) * kMaxLogLists;
+        hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<uint32_t>(std::min<uint64_t>((cells + kBlock - 1) / kBlock, 4096))),
+                           dim3(kBlock), 0, s->stream, a.log_end, cells, 0u);
+    }
     return 0;
 }
 
