@@ -1922,6 +1922,9 @@ __global__ __launch_bounds__(kBlock) void k_end_level(LevelArgs a, uint32_t d) {
         const Compact& c = sp ? a.c1 : a.c0;
         const uint32_t ng = sp ? ng1 : ng0, nw = sp ? nw1 : nw0, T = sp ? T1 : T0;
         const uint32_t g0 = tt * kEndTileGroups;
+        // the tile's flag words, in flight while wave 0 looks back
+        const uint32_t word = g0 * kGroupWords + threadIdx.x;
+        const uint64_t bits = word < nw ? c.flags[word] : 0ull;
         if (wid == 0) {  // the tile's group counts: in-tile offsets, count, states; then the look-back
             const uint32_t g = g0 + lane;
             const uint32_t cg = g < ng ? c.gcnt[g] : 0u;
@@ -1984,8 +1987,6 @@ __global__ __launch_bounds__(kBlock) void k_end_level(LevelArgs a, uint32_t d) {
         }
         __syncthreads();
         // the tile's words, one per thread, to their list positions
-        const uint32_t word = g0 * kGroupWords + threadIdx.x;
-        const uint64_t bits = word < nw ? c.flags[word] : 0ull;
         const uint32_t pc = __popcll(bits);
         uint32_t incl = pc;
         for (int off = 1; off < kGroupWords; off <<= 1) {
