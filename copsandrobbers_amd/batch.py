@@ -74,6 +74,33 @@ def clock_ns():
     return int(_lib().cr_clock_ns())
 
 
+class Packed:
+    """Graphs packed once for repeated launches: n int32[], k int32[], the
+    concatenated adjacency matrices and their offsets (what cr_batch_solve
+    reads)."""
+    __slots__ = ("ns", "ks", "adj", "offs")
+
+    def __init__(self, graphs):
+        self.ns, self.ks, self.adj, self.offs = _pack(graphs)
+
+    def __len__(self):
+        return len(self.ns)
+
+
+def pack(graphs):
+    """graphs: list of (adj uint8[n, n], k) -> Packed."""
+    return Packed(graphs)
+
+
+# GraphResult as a numpy record (results="array" returns the launch's results
+# in this layout without a Python object per graph)
+RESULT_DTYPE = np.dtype({"names": [f for f, _ in GraphResult._fields_],
+                         "formats": [np.uint64, np.int32, np.int32, np.int32, np.int32, np.uint64, (np.uint8, MAX_K),
+                                     np.uint64, np.uint64, np.uint64, np.uint64, np.float64],
+                         "offsets": [getattr(GraphResult, f).offset for f, _ in GraphResult._fields_],
+                         "itemsize": ctypes.sizeof(GraphResult)})
+
+
 def _pack(graphs):
     """(n int32[], k int32[], concatenated matrices, offsets uint64[])."""
     ns = np.array([np.asarray(a).shape[0] for a, _ in graphs], np.int32)
@@ -140,11 +167,13 @@ class Batch:
     def __exit__(self, *exc):
         self.close()
 
-    def solve(self, graphs, tables=False, raise_on_error=True):
-        """graphs: list of (adj uint8[n, n], k). Returns a list of result dicts
-        (plus the list of state tables when tables=True)."""
+    def solve(self, graphs, tables=False, raise_on_error=True, results="dicts"):
+        """graphs: list of (adj uint8[n, n], k), or a Packed. Returns the
+        results as a list of dicts, or with results="array" as a RESULT_DTYPE
+        record array (positions hold MAX_K entries, the first k used); plus
+        the list of state tables when tables=True."""
         count = len(graphs)
-        ns, ks, adj, offs = _pack(graphs)
+        ns, ks, adj, offs = (graphs.ns, graphs.ks, graphs.adj, graphs.offs) if isinstance(graphs, Packed) else _pack(graphs)
         res = (GraphResult * max(count, 1))()
         tab = None
         toffs = None
@@ -164,6 +193,10 @@ class Batch:
                               toffs.ctypes.data_as(u64p) if toffs is not None else None)
         if rc != 0 and (raise_on_error or rc != -1):
             raise SolverError(rc, L.cr_last_error().decode(errors="replace"))
+        if results == "array":
+            arr = np.frombuffer(res, dtype=RESULT_DTYPE, count=count).copy()
+            return (arr, [tab[int(toffs[i]):int(toffs[i]) + int(arr[i]["table_bytes"])] for i in range(count)]) \
+                if tables else arr
         out = [to_dict(res[i]) for i in range(count)]
         for i, r in enumerate(out):
             r["positions"] = r["positions"][:int(ks[i])]

@@ -2340,14 +2340,18 @@ struct MoveArgs {
 
 template <int K>
 __global__ __launch_bounds__(kWave) void k_count_retro(GraphView gv, LevelArgs a, uint32_t slots,
-                                                       uint32_t shift, int32_t mode, MoveArgs mv) {
+                                                       uint32_t shift, int32_t mode, MoveArgs mv,
+                                                       unsigned long long* ghs) {
     __shared__ uint32_t fill_pos;
     __shared__ unsigned long long hist[2][128];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i >> 7][i & 127] = 0;
     extern __shared__ __align__(16) uint8_t lds[];
     load_image(lds, gv.image, gv.image_bytes);
-    // slot = (row + 1) << 32 | rank: a slot tagged with another row counts as empty
-    unsigned long long* hs = reinterpret_cast<unsigned long long*>(lds + ((gv.image_bytes + 15) & ~15u));
+    // slot = (row + 1) << 32 | rank: a slot tagged with another row counts as
+    // empty. The set is in LDS, or (ghs) in a per-workgroup slice of device
+    // memory where the product of the move counts makes it too large for LDS.
+    unsigned long long* hs = ghs ? ghs + static_cast<uint64_t>(blockIdx.x) * slots
+                                 : reinterpret_cast<unsigned long long*>(lds + ((gv.image_bytes + 15) & ~15u));
     const uint32_t* rank_tab = reinterpret_cast<const uint32_t*>(lds);
     const uint32_t* binom = reinterpret_cast<const uint32_t*>(lds + gv.off_binom);
     const uint16_t* in_off = reinterpret_cast<const uint16_t*>(lds + gv.off_in_off);
@@ -2954,15 +2958,15 @@ struct LaunchVerdict {
 
 template <int K>
 struct LaunchRetro {
-    static int run(cr_solver* s, LevelArgs a, uint32_t slots, int32_t mode, MoveArgs mv) {
-        const uint32_t lds = ((s->gv.image_bytes + 15) & ~15u) + slots * 8;
+    static int run(cr_solver* s, LevelArgs a, uint32_t slots, int32_t mode, MoveArgs mv, unsigned long long* ghs,
+                   uint32_t blocks) {
+        const uint32_t lds = ((s->gv.image_bytes + 15) & ~15u) + (ghs ? 0u : slots * 8);
         uint32_t shift = 32;
         for (uint32_t v = slots; v > 1; v >>= 1) --shift;
         hipFuncSetAttribute(reinterpret_cast<const void*>(k_count_retro<K>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(s->C, 8192));
         hipLaunchKernelGGL(k_count_retro<K>, dim3(blocks), dim3(kWave), lds, s->stream, s->gv, a, slots, shift, mode,
-                           mv);
+                           mv, ghs);
         return 0;
     }
 };
@@ -3728,6 +3732,27 @@ int do_loop(cr_solver* s) {
     return 0;
 }
 
+// A temporary device allocation, freed when it goes out of scope (hipFree
+// waits for the work that uses it).
+struct DeviceBuffer {
+    void* p = nullptr;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() {
+        if (p) hipFree(p);
+    }
+    int alloc(uint64_t bytes) {
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            return fail(CR_ERR_TOO_LARGE, "%llu bytes of device scratch", static_cast<unsigned long long>(bytes));
+        }
+        return 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
 int count_moves(cr_solver* s, int32_t mode, unsigned long long* r, MoveArgs mv = MoveArgs{}) {
     HIP_TRY(hipSetDevice(s->device));
     uint64_t prod = 1;
@@ -3736,13 +3761,20 @@ int count_moves(cr_solver* s, int32_t mode, unsigned long long* r, MoveArgs mv =
     uint32_t slots = 64;
     while (slots < 2 * prod) slots <<= 1;
     const uint64_t lds = ((s->gv.image_bytes + 15) & ~15u) + static_cast<uint64_t>(slots) * 8;
-    if (lds > 160 * 1024)
-        return fail(CR_ERR_UNSUPPORTED, "move count needs %llu bytes of LDS", static_cast<unsigned long long>(lds));
+    uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(s->C, 8192));
+    DeviceBuffer ghs;  // the dedup sets in device memory when they do not fit in LDS (counts only)
+    if (lds > 160 * 1024) {
+        // the CSR build (modes 2, 3) is an optimisation and goes without
+        if (mode >= 2 || slots > (1u << 24))
+            return fail(CR_ERR_UNSUPPORTED, "move count needs %llu bytes of LDS", static_cast<unsigned long long>(lds));
+        blocks = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(blocks, (1ull << 30) / (slots * 8ull))));
+        if (int rc = ghs.alloc(static_cast<uint64_t>(blocks) * slots * 8)) return rc;
+    }
     static const unsigned long long zero[256] = {};
     HIP_TRY(hipMemcpyAsync(s->d_ctl->retro, zero, 3 * sizeof(unsigned long long), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_ctl->hist, zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
     LevelArgs a = level_args(s);
-    if (int rc = dispatch_k<LaunchRetro>(s->k, s, a, slots, mode, mv)) return rc;
+    if (int rc = dispatch_k<LaunchRetro>(s->k, s, a, slots, mode, mv, ghs.as<unsigned long long>(), blocks)) return rc;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(r, s->d_ctl->retro, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));

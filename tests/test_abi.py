@@ -97,3 +97,24 @@ def test_workload_graphs_regenerate(oracle):
         assert np.array_equal(adj, make()), stem
         assert np.array_equal(adj, adj.T)
         assert workloads.is_connected(adj)
+
+
+def test_batch_result_record_layout_matches_the_struct():
+    """The record dtype of Batch.solve(..., results="array") reads
+    cr_graph_result as laid out by the C ABI (no device needed)."""
+    import ctypes
+
+    from copsandrobbers_amd import batch as crb
+    assert crb.RESULT_DTYPE.itemsize == ctypes.sizeof(crb.GraphResult)
+    res = (crb.GraphResult * 3)()
+    res[2].ticket, res[2].status, res[2].win, res[2].plys = 11, -5, 1, 7
+    res[2].best_cid, res[2].base_captures, res[2].won_states = 1 << 40, 3, 99
+    res[2].table_hash, res[2].table_bytes, res[2].latency_ms = (1 << 63) + 5, 4096, 2.25
+    res[2].positions[0], res[2].positions[3] = 4, 200
+    a = np.frombuffer(res, dtype=crb.RESULT_DTYPE, count=3)
+    r = a[2]
+    assert (int(r["ticket"]), int(r["status"]), int(r["win"]), int(r["plys"])) == (11, -5, 1, 7)
+    assert (int(r["best_cid"]), int(r["base_captures"]), int(r["won_states"])) == (1 << 40, 3, 99)
+    assert (int(r["table_hash"]), int(r["table_bytes"]), float(r["latency_ms"])) == ((1 << 63) + 5, 4096, 2.25)
+    assert list(r["positions"][:4]) == [4, 0, 0, 200]
+    assert int(a[0]["ticket"]) == 0

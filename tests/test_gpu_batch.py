@@ -152,6 +152,27 @@ def test_batch_matches_single_graph_solves(batch, solver, oracle):
             assert r["plys"] == verdict["plys"] and r["positions"] == verdict["positions"], i
 
 
+def test_packed_inputs_and_array_results_equal_the_dicts(batch):
+    """Graphs packed once and results as a record array (the form the bench
+    times) give the same results and tables as the list-of-dicts form."""
+    graphs = random_graphs(24, seed_base=7000)
+    say("batch packed", len(graphs))
+    res, tabs = batch.solve(graphs, tables=True)
+    packed = crb.pack(graphs)
+    for _ in range(2):  # the packed arrays are reused
+        arr, tabs2 = batch.solve(packed, tables=True, results="array")
+        assert len(arr) == len(graphs)
+        for i, r in enumerate(res):
+            k = graphs[i][1]
+            for f in ("status", "plys", "n_passes", "best_cid", "base_captures", "won_states", "table_hash",
+                      "table_bytes"):
+                assert int(arr[i][f]) == r[f], (i, f)
+            assert bool(arr[i]["win"]) == r["win"], i
+            assert tuple(int(x) for x in arr[i]["positions"][:k]) == r["positions"], i
+            assert np.array_equal(tabs2[i], tabs[i]), i
+    assert len(batch.solve(crb.pack([]), results="array")) == 0
+
+
 def test_batch_reports_invalid_graphs_and_solves_the_rest(batch):
     good, _ = workloads.er_graph(9, 0.4, 1)
     iso = good.copy()

@@ -104,6 +104,24 @@ def test_levels_and_retro_edges_match_oracle(solver, oracle):
         assert got == (e_cop, e_rob, int((states & 1).sum()))
 
 
+def test_move_counts_of_dense_graphs_match_oracle(solver, oracle):
+    """Move products too large for the LDS dedup set (degree 22, k = 3: a
+    set of 2 * 22^3 rounded up to 32768 slots, 256 KiB) are counted through
+    the device-memory set: transitions and retro-edges equal the oracle's."""
+    n = 24
+    adj = np.ones((n, n), np.uint8) - np.eye(n, dtype=np.uint8)
+    for i in range(0, n, 2):  # minus a perfect matching
+        adj[i, i + 1] = adj[i + 1, i] = 0
+    say("dense move counts")
+    solver.load_graph_matrix(adj, 3, 1)
+    solver.build_aux_graph()
+    solver.initialize_captures()
+    solver.main_loop()
+    states = solver.states()
+    assert solver.count_transitions() == int(oracle.transitions(adj, 3)[0][-1])
+    assert solver.retro_edges() == oracle.retro_edges(adj, 3, states)
+
+
 def test_er200_k2_matches_oracle(solver, oracle):
     path = matrix("er200_p0.03_s1")
     n, adj = oracle.load_matrix(path)
