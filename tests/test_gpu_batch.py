@@ -152,6 +152,24 @@ def test_batch_matches_single_graph_solves(batch, solver, oracle):
             assert r["plys"] == verdict["plys"] and r["positions"] == verdict["positions"], i
 
 
+def test_benchmarked_batch_equals_single_solves_and_oracle(batch, solver, oracle):
+    """The 4096 graphs the batch benchmark solves (workloads.batch_graphs(4096),
+    one packed launch): every table's hash, pass count, verdict and capture
+    count equals the single-graph solve's, and every 64th graph's table equals
+    the oracle's byte for byte."""
+    graphs = workloads.batch_graphs(4096)
+    say("benchmarked batch", len(graphs))
+    arr = batch.solve(crb.pack(graphs), results="array")
+    for i, (adj, k) in enumerate(graphs):
+        base, passes, verdict, states, won = single(solver, adj, k)
+        assert int(arr["table_hash"][i]) == table_hash(states), i
+        assert batch.pass_counts(i) == passes, i
+        assert int(arr["base_captures"][i]) == base and int(arr["won_states"][i]) == won, i
+        assert bool(arr["win"][i]) == (verdict is not None), i
+        if i % 64 == 0:
+            assert np.array_equal(states, oracle.v1_solve(adj, k, 1).states), i
+
+
 def test_packed_inputs_and_array_results_equal_the_dicts(batch):
     """Graphs packed once and results as a record array (the form the bench
     times) give the same results and tables as the list-of-dicts form."""
