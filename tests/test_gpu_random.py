@@ -1,7 +1,8 @@
 """Randomised parity of the plain solve (v1, p = 1) with the oracle: random small
 graphs (6..40 vertices, 1..4 cops), and graphs of 65..130 vertices with 1..2
 cops whose robber rows take two or three 64-bit words, under random speed
-settings, none of which may change a table. Fixed seeds; every case prints its
+settings, none of which may change a table; and 66- and 80-vertex graphs
+(two-word rows) with 3 cops under the pull's kernel variants. Fixed seeds; every case prints its
 graph size and settings before it runs."""
 import numpy as np
 import pytest
@@ -83,5 +84,35 @@ def test_random_wide_rows_and_settings_match_oracle(oracle, seed):
             s.main_loop()
             want = oracle.v1_levelsync(adj, k, nthreads=8)[0]
             assert np.array_equal(s.states(), want), (seed, case, n, k, tuning)
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("seed", [31])
+def test_wide_rows_with_three_cops_match_oracle(oracle, seed):
+    """Two-word robber rows at k = 3 (66 and 80 vertices): the default settings,
+    every level pulled (stage 1 by tiles of consecutive multisets, the last two
+    stages in one k_pull_pair launch, or stage by stage) and the depth log with
+    the fused last stage; each table equals the oracle's."""
+    rng = np.random.default_rng(seed)
+    cases = [({}, 66), ({"pull_scale": 0}, 66), ({"pull_scale": 0, "pull_pair": 0}, 80),
+             ({"pull_scale": 0, "depth_log_min": 0}, 80)]
+    s = cra.Solver(0)
+    try:
+        for case, (tuning, n) in enumerate(cases):
+            while True:
+                adj = np.triu((rng.random((n, n)) < 0.06).astype(np.uint8), 1)
+                adj = adj | adj.T
+                if adj.sum(1).min() > 0:
+                    break
+            print(f"wide k=3 case {seed}/{case}: n={n} {tuning}", flush=True)
+            s.set_tuning(pull_scale=-1, pull_pair=1, depth_log_min=1 << 24)
+            s.set_tuning(**tuning)
+            s.load_graph_matrix(adj, 3, 1)
+            s.build_aux_graph()
+            s.initialize_captures()
+            s.main_loop()
+            want = oracle.v1_levelsync(adj, 3, nthreads=8)[0]
+            assert np.array_equal(s.states(), want), (seed, case, n, tuning)
     finally:
         s.close()
