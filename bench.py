@@ -30,7 +30,10 @@ Algorithmic bytes per unit (scalar-record model): 0.5 B per cop retro-edge
 served, 2 B per robber retro-edge (8-bit escape counter read + written) for
 the robber step, 1.25 B per newly won cop-turn state (depth byte + frontier
 bit) for the finaliser. solve_roofline is the whole solve's algorithmic bytes
-(B_alg) over the step time.
+(B_alg) over the step time. roofline.traffic is the dominant kernel's measured
+HBM bytes per dispatch (rocprofv3 FETCH_SIZE/WRITE_SIZE in profiles/, scaled by
+a device copy of known size), and roofline.traffic_rate those bytes over the
+profiled dispatch time: what the kernel moves, beside the model's bytes.
 
 cpu_baseline: the reference solver (oracle/_ref/k_cops_visibility, built from
 the reference's own sources) on one host core, on a bounded sample of the
@@ -81,21 +84,41 @@ KERNEL_NAMES = {"cop_push": "k_cop_push", "sep_stage": "k_pull_tile + k_pull_til
                 "mx_planes": "k_mx_planes", "mx_mark": "k_mx_mark"}
 
 
-def measured_traffic(workload, timer_name):
-    """HBM bytes per dispatch of a kernel from the committed rocprofv3 PMC summary
-    (profiles/<workload>_traffic.json, FETCH_SIZE/WRITE_SIZE scaled by the copy calibration)."""
+def traffic_profile(workload, timer_name):
+    """(HBM bytes, duration in us) per dispatch of a kernel from the committed rocprofv3
+    summaries (profiles/<workload>_traffic.json: FETCH_SIZE/WRITE_SIZE scaled by the copy
+    calibration, durations from the kernel trace of the same command), or (None, None)."""
     path = os.path.join(REPO, "profiles", f"{workload}_traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         data = json.load(f)
     kernels = data.get("kernels", {})
-    # a timer can cover several device kernels (the pull stages): dispatch-weighted mean
+    # a timer can cover several device kernels (the pull stages): dispatch-weighted means
     found = [kernels[n] for n in KERNEL_NAMES.get(timer_name, timer_name).split(" + ") if n in kernels]
     if not found:
-        return None
+        return None, None
     calls = sum(k.get("dispatches", 1) for k in found)
-    return sum(k["hbm_bytes"] * k.get("dispatches", 1) for k in found) / calls
+    hbm = sum(k["hbm_bytes"] * k.get("dispatches", 1) for k in found) / calls
+    timed = [k for k in found if k.get("avg_us")]
+    us = (sum(k["avg_us"] * k.get("dispatches", 1) for k in timed) / sum(k.get("dispatches", 1) for k in timed)
+          if timed else None)
+    return hbm, us
+
+
+def measured_traffic(workload, timer_name):
+    """HBM bytes per dispatch of a kernel (traffic_profile)."""
+    return traffic_profile(workload, timer_name)[0]
+
+
+def traffic_rate(workload, timer_name):
+    """Measured HBM bytes per dispatch over the profiled dispatch time, GB/s and as a
+    fraction of the 8 TB/s peak: what the kernel moves, beside the model's bytes."""
+    hbm, us = traffic_profile(workload, timer_name)
+    if not hbm or not us:
+        return None
+    rate = hbm / (us * 1e-6)
+    return {"GBps": rate / 1e9, "frac": rate / HBM_PEAK}
 
 
 def matrix_path(stem):
@@ -272,6 +295,7 @@ def bench_macro(args):
         achieved = per_launch / (ms / 1e3 / launches)
         rl = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
               "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.workload, dominant),
+              "traffic_rate": traffic_rate(args.workload, dominant),
               "kernel": KERNEL_NAMES[dominant], "alg_bytes_per_launch": per_launch,
               "avg_launch_us": ms * 1e3 / launches}
     out = {
@@ -435,7 +459,8 @@ def run_batch(steps, warmup, plain_sample, serve_load, autotune_now=False, db=No
         "graphs": len(graphs), "states": states, "table_bytes": states, "workgroups": grid, "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": alg / dev_s / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": alg / dev_s / HBM_PEAK,
-                     "traffic": measured_traffic("batch4096", "k_batch_solve"), "kernel": "k_batch_solve",
+                     "traffic": measured_traffic("batch4096", "k_batch_solve"),
+                     "traffic_rate": traffic_rate("batch4096", "k_batch_solve"), "kernel": "k_batch_solve",
                      "alg_bytes_per_launch": alg, "avg_launch_us": dev_s * 1e6,
                      "note": "achieved counts the scalar-record model's bytes (0.5 B per cop retro-edge, 2 B "
                              "per robber retro-edge); the per-graph factorised pull does not visit each cop "
@@ -627,6 +652,7 @@ def bench_plain(args):
         achieved = bytes_per_launch / avg_s
         rl = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
               "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.workload, dominant),
+              "traffic_rate": traffic_rate(args.workload, dominant),
               "kernel": KERNEL_NAMES.get(dominant, dominant),
               "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
               "launches_per_step": launches}
