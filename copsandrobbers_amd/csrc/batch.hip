@@ -112,19 +112,144 @@ __device__ __forceinline__ uint32_t small_binom(int x, int y) {
     return static_cast<uint32_t>(c);
 }
 
+// Multisets of one or two vertices: rank <-> values without the binomial
+// search (lexicographic order over [0, n); pairs from a per-graph table).
+constexpr int kPairTab = (kMaxBN * (kMaxBN + 1)) / 2;
+__device__ __forceinline__ uint32_t rank_pair_n(uint32_t a, uint32_t b, uint32_t n) {
+    return a * n - (a * (a - 1)) / 2 + (b - a);
+}
+// Multisets of three vertices from a per-graph table where the graph's
+// C(n+2, 3) fits (n <= 48), else by the binomial search.
+constexpr uint32_t kTripTab = 19600;  // C(50, 3)
+template <int J>
+__device__ __forceinline__ void unrank_small(uint32_t r, int n, const uint32_t* s_binom, int Mimg,
+                                             const uint16_t* s_pair, const uint32_t* s_trip, uint32_t* out) {
+    if constexpr (J == 1) {
+        out[0] = r;
+    } else if constexpr (J == 2) {
+        const uint32_t p = s_pair[r];
+        out[0] = p & 0xffu;
+        out[1] = p >> 8;
+    } else if constexpr (J == 3) {
+        if (s_trip != nullptr) {
+            const uint32_t t = s_trip[r];
+            out[0] = t & 0xffu;
+            out[1] = (t >> 8) & 0xffu;
+            out[2] = t >> 16;
+        } else {
+            unrank_j(r, J, n, s_binom, Mimg, out);
+        }
+    } else if constexpr (J > 3) {
+        unrank_j(r, J, n, s_binom, Mimg, out);
+    }
+}
+template <int J>
+__device__ __forceinline__ uint32_t rank_small(const uint32_t* t, int n, const uint32_t* s_binom, int Mimg) {
+    if constexpr (J == 0) {
+        return 0;
+    } else if constexpr (J == 1) {
+        return t[0];
+    } else if constexpr (J == 2) {
+        return rank_pair_n(t[0], t[1], static_cast<uint32_t>(n));
+    } else {
+        return rank_j(t, J, n, s_binom, Mimg);
+    }
+}
+
+// Stage I of a graph's factorised pull: H_I[A][B] = OR_{v in N(max A)}
+// H_{I-1}[A - max A][B + v] (H_0 = fr1; stage K writes the candidates masked
+// with ~won0), rows o = rank(A) * cnt(K - I) + rank(B), lanes over rows.
+template <int K, int I>
+__device__ __forceinline__ void pull_stage(const uint64_t* __restrict__ in, uint64_t* __restrict__ dst, const uint32_t* cnt,
+                                           int n, int Mimg, const uint32_t* s_binom, const uint16_t* s_pair,
+                                           const uint32_t* s_trip, const uint64_t* s_mask,
+                                           const uint64_t* __restrict__ won0, uint64_t full) {
+    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    const uint32_t cb = cnt[K - I], cb_in = cnt[K - I + 1], rows = cnt[I] * cb;
+    for (uint32_t o = tid; o < rows; o += T) {
+        const uint32_t ra = o / cb, rb = o - ra * cb;
+        uint32_t A[K], B[K + 1];
+        unrank_small<I>(ra, n, s_binom, Mimg, s_pair, s_trip, A);
+        unrank_small<K - I>(rb, n, s_binom, Mimg, s_pair, s_trip, B);
+        const uint64_t row_in = static_cast<uint64_t>(rank_small<I - 1>(A, n, s_binom, Mimg)) * cb_in;
+        uint64_t acc = 0, bits = s_mask[A[I - 1]];
+        while (bits) {
+            // ranks of up to kMlp successor rows first, then their loads together
+            uint32_t idx[kMlp];
+            int m = 0;
+#pragma unroll
+            for (int u = 0; u < kMlp; ++u) {
+                if (bits == 0) break;
+                const uint32_t v = static_cast<uint32_t>(__builtin_ctzll(bits));
+                bits &= bits - 1;
+                uint32_t M2[K + 1];
+                int pos = 0;
+                bool placed = false;
+#pragma unroll
+                for (int t = 0; t < K - I; ++t) {
+                    if (!placed && v <= B[t]) {
+                        M2[pos++] = v;
+                        placed = true;
+                    }
+                    M2[pos++] = B[t];
+                }
+                if (!placed) M2[pos++] = v;
+                idx[u] = rank_small<K - I + 1>(M2, n, s_binom, Mimg);
+                m = u + 1;
+            }
+            uint64_t got[kMlp];
+#pragma unroll
+            for (int u = 0; u < kMlp; ++u) got[u] = u < m ? in[row_in + idx[u]] : 0ull;
+#pragma unroll
+            for (int u = 0; u < kMlp; ++u) acc |= got[u];
+        }
+        dst[o] = I == K ? (acc & ~won0[o] & full) : acc;
+    }
+    __syncthreads();
+}
+
 template <int K>
 __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ masks, uint8_t* __restrict__ tables,
                             BatchOut* __restrict__ out, uint64_t* __restrict__ scratch, uint64_t* s_mask,
-                            uint32_t* s_binom, unsigned long long* s_red) {
+                            uint32_t* s_binom, unsigned long long* s_red, uint16_t* s_pair, uint32_t* s_trip) {
     const int n = static_cast<int>(g.n), Mimg = n + K - 1;
     const uint32_t C = g.C, T = blockDim.x, tid = threadIdx.x;
     const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
     for (int i = tid; i < n; i += T) s_mask[i] = masks[g.mask_off + i];
     for (int i = tid; i < (K + 1) * (Mimg + 1); i += T) s_binom[i] = small_binom(i % (Mimg + 1), i / (Mimg + 1));
+    // the pair table: row a of the lexicographic order holds (a, a..n-1)
+    for (int a = 0; a < n; ++a) {
+        const uint32_t base = rank_pair_n(a, a, n);
+        for (int b = a + static_cast<int>(tid); b < n; b += T) s_pair[base + (b - a)] = static_cast<uint16_t>(a | (b << 8));
+    }
     __syncthreads();
     uint32_t cnt[K + 1];
 #pragma unroll
     for (int j = 0; j <= K; ++j) cnt[j] = s_binom[j * (Mimg + 1) + (n + j - 1)];  // C(n+j-1, j)
+    // the triple table (K >= 3 and C(n+2, 3) <= kTripTab): each thread fills a
+    // contiguous run, one search for its first entry and lexicographic steps after
+    const uint32_t* trip = nullptr;
+    if constexpr (K >= 3) {
+        if (cnt[3] <= kTripTab) {
+            const uint32_t per = (cnt[3] + T - 1) / T, e0 = tid * per, e1 = min(cnt[3], e0 + per);
+            if (e0 < e1) {
+                uint32_t t3[3];
+                unrank_j(e0, 3, n, s_binom, Mimg, t3);
+                for (uint32_t e = e0; e < e1; ++e) {
+                    s_trip[e] = t3[0] | (t3[1] << 8) | (t3[2] << 16);
+                    if (t3[2] + 1 < static_cast<uint32_t>(n)) {
+                        ++t3[2];
+                    } else if (t3[1] + 1 < static_cast<uint32_t>(n)) {
+                        t3[1] = t3[2] = t3[1] + 1;
+                    } else {
+                        t3[0] = t3[1] = t3[2] = t3[0] + 1;
+                    }
+                }
+            }
+            __syncthreads();
+            trip = s_trip;
+        }
+    }
     uint64_t* won0 = scratch;
     uint64_t* won1 = won0 + C;
     uint64_t* fr1 = won1 + C;
@@ -138,7 +263,7 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
     unsigned long long base = 0, unused = 0;
     for (uint32_t c = tid; c < C; c += T) {
         uint32_t cops[K];
-        unrank_j(c, K, n, s_binom, Mimg, cops);
+        unrank_small<K>(c, n, s_binom, Mimg, s_pair, trip, cops);
         uint64_t occ = 0;
 #pragma unroll
         for (int j = 0; j < K; ++j) occ |= 1ull << cops[j];
@@ -166,51 +291,10 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
         const bool do_cop = prev1 != 0, do_rob = prev0 != 0;
         // cop turn: factorised pull from the robber frontier
         if (do_cop) {
-#pragma unroll
-            for (int i = 1; i <= K; ++i) {
-                const uint64_t* in = i == 1 ? fr1 : hbuf[(i - 2) & 1];
-                uint64_t* dst = i == K ? nx0 : hbuf[(i - 1) & 1];
-                const uint32_t cb = cnt[K - i], cb_in = cnt[K - i + 1], rows = cnt[i] * cb;
-                for (uint32_t o = tid; o < rows; o += T) {
-                    const uint32_t ra = o / cb, rb = o - ra * cb;
-                    uint32_t A[K], B[K + 1];
-                    unrank_j(ra, i, n, s_binom, Mimg, A);
-                    unrank_j(rb, K - i, n, s_binom, Mimg, B);
-                    const uint64_t row_in = static_cast<uint64_t>(rank_j(A, i - 1, n, s_binom, Mimg)) * cb_in;
-                    uint64_t acc = 0, bits = s_mask[A[i - 1]];
-                    while (bits) {
-                        // ranks of up to kMlp successor rows first, then their loads together
-                        uint32_t idx[kMlp];
-                        int m = 0;
-#pragma unroll
-                        for (int u = 0; u < kMlp; ++u) {
-                            if (bits == 0) break;
-                            const uint32_t v = static_cast<uint32_t>(__builtin_ctzll(bits));
-                            bits &= bits - 1;
-                            uint32_t M2[K + 1];
-                            int pos = 0;
-                            bool placed = false;
-                            for (int t = 0; t < K - i; ++t) {
-                                if (!placed && v <= B[t]) {
-                                    M2[pos++] = v;
-                                    placed = true;
-                                }
-                                M2[pos++] = B[t];
-                            }
-                            if (!placed) M2[pos++] = v;
-                            idx[u] = rank_j(M2, K - i + 1, n, s_binom, Mimg);
-                            m = u + 1;
-                        }
-                        uint64_t got[kMlp];
-#pragma unroll
-                        for (int u = 0; u < kMlp; ++u) got[u] = u < m ? in[row_in + idx[u]] : 0ull;
-#pragma unroll
-                        for (int u = 0; u < kMlp; ++u) acc |= got[u];
-                    }
-                    dst[o] = i == K ? (acc & ~won0[o] & full) : acc;
-                }
-                __syncthreads();
-            }
+            pull_stage<K, 1>(fr1, K == 1 ? nx0 : hbuf[0], cnt, n, Mimg, s_binom, s_pair, trip, s_mask, won0, full);
+            if constexpr (K >= 2) pull_stage<K, 2>(hbuf[0], K == 2 ? nx0 : hbuf[1], cnt, n, Mimg, s_binom, s_pair, trip, s_mask, won0, full);
+            if constexpr (K >= 3) pull_stage<K, 3>(hbuf[1], K == 3 ? nx0 : hbuf[0], cnt, n, Mimg, s_binom, s_pair, trip, s_mask, won0, full);
+            if constexpr (K >= 4) pull_stage<K, 4>(hbuf[0], nx0, cnt, n, Mimg, s_binom, s_pair, trip, s_mask, won0, full);
         }
         // robber turn: every move of r lands on a won cop-turn state (depth <= d)
         if (do_rob) {
@@ -297,6 +381,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
     __shared__ uint64_t s_mask[kMaxBN];
     __shared__ uint32_t s_binom[(kMaxBK + 1) * (kBinomCols + 1)];
     __shared__ unsigned long long s_red[2 * kMaxBlockWaves];
+    __shared__ uint16_t s_pair[kPairTab];
+    __shared__ uint32_t s_trip[kTripTab];
     __shared__ uint32_t s_item;
     uint64_t* slot = scratch + static_cast<uint64_t>(blockIdx.x) * slot_words;
     for (;;) {
@@ -308,10 +394,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
         const uint32_t gi = order[item];
         const BatchGraph g = graphs[gi];
         switch (g.k) {
-            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red); break;
-            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red); break;
-            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red); break;
-            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red); break;
+            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_trip); break;
+            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_trip); break;
+            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_trip); break;
+            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_trip); break;
             default: break;
         }
         __syncthreads();
