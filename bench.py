@@ -463,12 +463,11 @@ def run_batch(steps, warmup, plain_sample, serve_load, autotune_now=False, db=No
                      "traffic_rate": traffic_rate("batch4096", "k_batch_solve"), "kernel": "k_batch_solve",
                      "alg_bytes_per_launch": alg, "avg_launch_us": dev_s * 1e6,
                      "note": "achieved counts the scalar-record model's bytes (0.5 B per cop retro-edge, 2 B "
-                             "per robber retro-edge); the per-graph factorised pull does not visit each cop "
-                             "retro-edge, so on these dense small graphs the model's bytes approach the launch's "
-                             "time at peak while its measured traffic is lower. traffic is the per-dispatch mean "
-                             "over the profiled run (full launches and the server's micro-batches); a full "
-                             "4096-graph launch moves ~27 GB: the work planes of the resident workgroups (up to "
-                             "1.7 MB per slot) do not fit the caches"},
+                             "per robber retro-edge), as for the single-graph line. Here it can exceed the HBM peak: "
+                             "each graph's work planes stay in the caches for most of its levels and the per-graph "
+                             "factorised pull does not visit each cop retro-edge, so those bytes are not moved. "
+                             "traffic and traffic_rate are what the launches move (rocprofv3, mean over the "
+                             "profiled run's full launches and the server's micro-batches)"},
         "plain_loop": {"graphs_per_s": plain, "sample": int(len(idx)), "speedup_batched": batched / plain},
         "serving": {"arrival_rate_per_s": rate, "requests": nreq,
                     "served_per_s": nreq / ((t_end - start) / 1e9),
