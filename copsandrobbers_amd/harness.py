@@ -1,17 +1,21 @@
-"""Regression harness with the reference's CSV contract.
+"""Sweep runner that writes the reference's regression CSV.
 
-Runs a solver executable over graphs x k x p, parses its stdout and writes
-one CSV row per run with the columns of util/test_k_cops_visibility.py:231-233
-(graph, n, k, p, est_state_mb, status, result, capture_plys, passes,
-base_caps, cache_hit, mem_mb, mainloop_s, solver_s, wall_s, exit_code,
-positions). The same stdout lines are parsed (:78-100), so the reference
-binaries and this package's CLI (bin/k_cops_visibility*) can be swept alike.
+A sweep solves every graph of a matrix folder for k = 1..max_k cops and
+visibility p = 1..max_p with a solver executable (the reference's binaries or
+this package's CLI, bin/k_cops_visibility*), one process per case, cheapest
+case first, and writes one CSV row per case. The columns are the reference
+harness's (util/test_k_cops_visibility.py:231-233), and the values come from
+the solver's stdout lines that harness reads (:78-100): verdict, start
+positions, capture plies, main-loop seconds, memory footprint, base captures,
+cache hit and the last pass number. Cases are costed, and cut at a state
+budget, by the reference's estimate n^k * n * 2p bytes (:72-74).
 
 Usage:
   python -m copsandrobbers_amd.harness --matrices tests/golden/matrices \\
       --exe copsandrobbers_amd/bin/k_cops_visibility --csv out.csv
 """
 import argparse
+import collections
 import csv
 import os
 import re
@@ -19,74 +23,110 @@ import subprocess
 import sys
 import time
 
-RE_RESULT = re.compile(r"RESULT:\s+(WIN|LOSS)")
-RE_POSITIONS = re.compile(r"Optimal Cop Start Positions:\s*\(([^)]*)\)")
-RE_CAPTURE = re.compile(r"Capture Time:\s*(\d+)\s*plys")
-RE_MAINLOOP = re.compile(r"Main Loop\s*-*=>\s*([\d.]+)\s*s")
-RE_FOOTPRINT = re.compile(r"Total Footprint\s*-*=>\s*(\d+)\s*B")
-RE_PASS = re.compile(r"Pass\s+(\d+):")
-RE_CACHE_HIT = re.compile(r"Loaded AuxGraph from cache")
-RE_BASECAPS = re.compile(r"Initialized\s+(\d+)\s+base captures")
+# Column order of the reference's CSV.
+CSV_COLUMNS = ["graph", "n", "k", "p", "est_state_mb", "status", "result", "capture_plys", "passes",
+               "base_caps", "cache_hit", "mem_mb", "mainloop_s", "solver_s", "wall_s", "exit_code",
+               "positions"]
 
-FIELDS = ["graph", "n", "k", "p", "est_state_mb", "status", "result", "capture_plys", "passes",
-          "base_caps", "cache_hit", "mem_mb", "mainloop_s", "solver_s", "wall_s", "exit_code",
-          "positions"]
-BIG_GRAPHS = {"scotlandyard-all", "scotlandyard-yellow", "scotlandyard-green", "scotlandyard-red",
-              "cycle100", "line100", "tree127"}
+# Graphs a sweep leaves out unless asked (minutes per case in the reference).
+SLOW_GRAPHS = frozenset({"scotlandyard-all", "scotlandyard-yellow", "scotlandyard-green", "scotlandyard-red",
+                         "cycle100", "line100", "tree127"})
 
 
-def node_count(path):
+def _first(pattern, convert, missing=""):
+    """A stdout reader: the first match of `pattern`, converted, or `missing`."""
+    rx = re.compile(pattern)
+
+    def read(text):
+        m = rx.search(text)
+        return convert(m.group(1)) if m else missing
+    return read
+
+
+def _last_pass(text):
+    found = re.findall(r"Pass\s+(\d+):", text)
+    return int(found[-1]) if found else ""
+
+
+# CSV column -> how its value is read from the solver's stdout.
+STDOUT_READERS = {
+    "result": _first(r"RESULT:\s+(WIN|LOSS)", str, missing="?"),
+    "positions": _first(r"Optimal Cop Start Positions:\s*\(([^)]*)\)", str.strip),
+    "capture_plys": _first(r"Capture Time:\s*(\d+)\s*plys", int),
+    "mainloop_s": _first(r"Main Loop\s*-*=>\s*([\d.]+)\s*s", float),
+    "mem_mb": _first(r"Total Footprint\s*-*=>\s*(\d+)\s*B", lambda b: round(int(b) / 1e6, 1)),
+    "base_caps": _first(r"Initialized\s+(\d+)\s+base captures", int),
+    "cache_hit": lambda text: "Loaded AuxGraph from cache" in text,
+    "passes": _last_pass,
+}
+
+
+def vertex_count(path):
+    """Vertices of an adjacency-matrix file: the length of its first row."""
     with open(path) as f:
         for line in f:
-            row = line.strip()
-            if row:
-                return len(row)
+            if line.strip():
+                return len(line.strip())
     return 0
 
 
-def est_state_bytes(n, k, p):
-    """The reference harness's cost model: n^k configs * n * 2p bytes."""
+def state_bytes_estimate(n, k, p):
+    """The reference's cost estimate of a case: n^k configurations x n robber
+    positions x 2p turns, one byte each."""
     return (n ** k) * n * (2 * p)
 
 
-def parse_output(text):
-    out = {}
-    m = RE_RESULT.search(text)
-    out["result"] = m.group(1) if m else "?"
-    m = RE_POSITIONS.search(text)
-    out["positions"] = m.group(1).strip() if m else ""
-    m = RE_CAPTURE.search(text)
-    out["capture_plys"] = int(m.group(1)) if m else ""
-    m = RE_MAINLOOP.search(text)
-    out["mainloop_s"] = float(m.group(1)) if m else ""
-    m = RE_FOOTPRINT.search(text)
-    out["mem_mb"] = round(int(m.group(1)) / 1e6, 1) if m else ""
-    m = RE_BASECAPS.search(text)
-    out["base_caps"] = int(m.group(1)) if m else ""
-    out["cache_hit"] = bool(RE_CACHE_HIT.search(text))
-    passes = RE_PASS.findall(text)
-    out["passes"] = int(passes[-1]) if passes else ""
-    out["solver_s"] = ""
-    return out
+def parse_stdout(text):
+    """The CSV values a solver's stdout carries (missing ones empty)."""
+    values = {column: read(text) for column, read in STDOUT_READERS.items()}
+    values["solver_s"] = ""
+    return values
 
 
-def run_one(exe, matrix_path, k, p, timeout=90.0, cwd=None):
-    cmd = [str(exe), str(matrix_path), str(k), str(p)]
-    t0 = time.perf_counter()
+def run_solver(exe, matrix_path, k, p, timeout=90.0, cwd=None):
+    """Runs `exe matrix k p` once; returns its CSV values with status, exit
+    code and wall time (status TIMEOUT when it does not finish in time)."""
+    start = time.perf_counter()
     try:
-        proc = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True, timeout=timeout)
+        proc = subprocess.run([str(exe), str(matrix_path), str(k), str(p)], cwd=cwd, capture_output=True,
+                              text=True, timeout=timeout)
     except subprocess.TimeoutExpired:
-        return {"status": "TIMEOUT", "wall_s": round(time.perf_counter() - t0, 2)}
-    parsed = parse_output(proc.stdout)
-    parsed["wall_s"] = round(time.perf_counter() - t0, 2)
-    parsed["exit_code"] = proc.returncode
-    got = parsed["result"] in ("WIN", "LOSS")
+        return {"status": "TIMEOUT", "wall_s": round(time.perf_counter() - start, 2)}
+    values = parse_stdout(proc.stdout)
+    values["wall_s"] = round(time.perf_counter() - start, 2)
+    values["exit_code"] = proc.returncode
+    verdict = values["result"] in ("WIN", "LOSS")
     if proc.returncode == 0:
-        parsed["status"] = "ok" if got else "no-result"
+        values["status"] = "ok" if verdict else "no-result"
     else:
-        parsed["status"] = ("ok-but-" if got else "") + f"exit{proc.returncode}"
-    parsed["stderr"] = proc.stderr
-    return parsed
+        values["status"] = ("ok-but-" if verdict else "") + f"exit{proc.returncode}"
+    values["stderr"] = proc.stderr
+    return values
+
+
+Case = collections.namedtuple("Case", "cost n path graph k p")
+
+
+def plan_sweep(matrices, max_k, max_p, state_budget, name_filters=(), include_slow=False):
+    """The cases of a sweep over the .txt matrices of a folder, cheapest first."""
+    cases = []
+    for fn in sorted(os.listdir(matrices)):
+        graph, ext = os.path.splitext(fn)
+        if ext != ".txt" or (name_filters and not any(f in graph for f in name_filters)):
+            continue
+        if graph in SLOW_GRAPHS and not include_slow:
+            continue
+        path = os.path.join(matrices, fn)
+        n = vertex_count(path)
+        if n < 2:
+            continue
+        for k in range(1, min(max_k, n - 1) + 1):
+            for p in range(1, max_p + 1):
+                cost = state_bytes_estimate(n, k, p)
+                if cost <= state_budget:
+                    cases.append(Case(cost, n, path, graph, k, p))
+    cases.sort(key=lambda c: c.cost)
+    return cases
 
 
 def main(argv=None):
@@ -98,45 +138,24 @@ def main(argv=None):
     ap.add_argument("--max-k", type=int, default=3)
     ap.add_argument("--max-p", type=int, default=3)
     ap.add_argument("--state-budget", type=float, default=1.5e9)
-    ap.add_argument("--graphs", default="")
-    ap.add_argument("--include-big", action="store_true")
+    ap.add_argument("--graphs", default="", help="comma-separated substrings of graph names to keep")
+    ap.add_argument("--include-big", action="store_true", help="also sweep the slow graphs")
     ap.add_argument("--cwd", default=None)
     args = ap.parse_args(argv)
-    filters = [s.strip() for s in args.graphs.split(",") if s.strip()]
-    jobs = []
-    for fn in sorted(os.listdir(args.matrices)):
-        if not fn.endswith(".txt"):
-            continue
-        stem = fn[:-4]
-        path = os.path.join(args.matrices, fn)
-        if filters and not any(f in stem for f in filters):
-            continue
-        if stem in BIG_GRAPHS and not args.include_big:
-            continue
-        n = node_count(path)
-        if n < 2:
-            continue
-        for k in range(1, args.max_k + 1):
-            if k >= n:
-                continue
-            for p in range(1, args.max_p + 1):
-                est = est_state_bytes(n, k, p)
-                if est <= args.state_budget:
-                    jobs.append((est, n, path, stem, k, p))
-    jobs.sort(key=lambda j: j[0])
+    filters = tuple(s.strip() for s in args.graphs.split(",") if s.strip())
+    exe = os.path.abspath(args.exe)
     rows = []
-    for est, n, path, stem, k, p in jobs:
-        res = run_one(os.path.abspath(args.exe), os.path.abspath(path), k, p, args.timeout, args.cwd)
-        row = {"graph": stem, "n": n, "k": k, "p": p, "est_state_mb": round(est / 1e6, 1)}
-        row.update(res)
+    for case in plan_sweep(args.matrices, args.max_k, args.max_p, args.state_budget, filters, args.include_big):
+        row = {"graph": case.graph, "n": case.n, "k": case.k, "p": case.p,
+               "est_state_mb": round(case.cost / 1e6, 1)}
+        row.update(run_solver(exe, os.path.abspath(case.path), case.k, case.p, args.timeout, args.cwd))
         rows.append(row)
-        print(f"{stem:<20} {n:>4} {k:>2} {p:>2} {row.get('result', ''):<6} {row.get('capture_plys', '')!s:>4} "
-              f"{row.get('passes', '')!s:>6} {row.get('status')}", flush=True)
+        print(f"{case.graph:<20} {case.n:>4} {case.k:>2} {case.p:>2} {row.get('result', ''):<6} "
+              f"{row.get('capture_plys', '')!s:>4} {row.get('passes', '')!s:>6} {row.get('status')}", flush=True)
     with open(args.csv, "w", newline="") as f:
-        w = csv.DictWriter(f, fieldnames=FIELDS, extrasaction="ignore")
-        w.writeheader()
-        for r in rows:
-            w.writerow(r)
+        writer = csv.DictWriter(f, fieldnames=CSV_COLUMNS, extrasaction="ignore")
+        writer.writeheader()
+        writer.writerows(rows)
     return 0
 
 

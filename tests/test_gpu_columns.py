@@ -145,7 +145,7 @@ def test_cli_reproduces_recorded_rows_p2_p3(tmp_path):
     from copsandrobbers_amd import harness
     for row in ROWS[::5]:
         say("columns cli", row["graph"], row["k"], row["p"])
-        res = harness.run_one(exe, matrix(row["graph"]), int(row["k"]), int(row["p"]), timeout=120,
+        res = harness.run_solver(exe, matrix(row["graph"]), int(row["k"]), int(row["p"]), timeout=120,
                               cwd=str(tmp_path))
         assert res["status"] == "ok", res
         for col in ("result", "capture_plys", "passes", "base_caps", "positions"):

@@ -183,7 +183,7 @@ def test_cli_v2_reproduces_recorded_rows(tmp_path):
     from copsandrobbers_amd import harness
     for row in FAST[::6]:
         say("v2 cli", row["graph"], row["k"], row["p"])
-        res = harness.run_one(exe, matrix(row["graph"]), int(row["k"]), int(row["p"]), timeout=120,
+        res = harness.run_solver(exe, matrix(row["graph"]), int(row["k"]), int(row["p"]), timeout=120,
                               cwd=str(tmp_path))
         assert res["status"] == "ok", res
         for col in ("result", "capture_plys", "passes", "base_caps", "positions"):

@@ -160,7 +160,7 @@ def test_cli_reproduces_recorded_rows(tmp_path):
     from copsandrobbers_amd import harness
     for row in ROWS[::3]:
         say("cli", row["graph"], row["k"])
-        res = harness.run_one(exe, matrix(row["graph"]), int(row["k"]), 1, timeout=120, cwd=str(tmp_path))
+        res = harness.run_solver(exe, matrix(row["graph"]), int(row["k"]), 1, timeout=120, cwd=str(tmp_path))
         assert res["status"] == "ok", res
         for col in ("result", "capture_plys", "passes", "base_caps", "positions"):
             assert str(res[col]) == row[col], (row["graph"], col)
@@ -176,7 +176,7 @@ def test_dropin_in_reference_supervisor_reproduces_recorded_rows(tmp_path):
     from copsandrobbers_amd import harness
     for row in ROWS[1::4]:
         say("dropin", row["graph"], row["k"])
-        res = harness.run_one(DROPIN, matrix(row["graph"]), int(row["k"]), 1, timeout=120, cwd=str(tmp_path))
+        res = harness.run_solver(DROPIN, matrix(row["graph"]), int(row["k"]), 1, timeout=120, cwd=str(tmp_path))
         assert res["status"] == "ok", res
         for col in ("result", "capture_plys", "passes", "base_caps", "positions"):
             assert str(res[col]) == row[col], (row["graph"], col)
