@@ -118,12 +118,14 @@ constexpr int kPairTab = (kMaxBN * (kMaxBN + 1)) / 2;
 __device__ __forceinline__ uint32_t rank_pair_n(uint32_t a, uint32_t b, uint32_t n) {
     return a * n - (a * (a - 1)) / 2 + (b - a);
 }
-// Multisets of three vertices from a per-graph table where the graph's
-// C(n+2, 3) fits (n <= 48), else by the binomial search.
-constexpr uint32_t kTripTab = 19600;  // C(50, 3)
+// Multisets of three vertices: s_t3[a] is the rank of (a, a, a), i.e. the
+// triples with a smaller first element, C(n+2, 3) - C(n-a+2, 3). The triples
+// (a, b, c) with first element a, in order, are the pairs (b, c) over [a, n),
+// and those are the suffix of the pair order from (a, a): a triple costs a
+// search over the n starts and one pair-table read.
 template <int J>
 __device__ __forceinline__ void unrank_small(uint32_t r, int n, const uint32_t* s_binom, int Mimg,
-                                             const uint16_t* s_pair, const uint32_t* s_trip, uint32_t* out) {
+                                             const uint16_t* s_pair, const uint32_t* s_t3, uint32_t* out) {
     if constexpr (J == 1) {
         out[0] = r;
     } else if constexpr (J == 2) {
@@ -131,14 +133,16 @@ __device__ __forceinline__ void unrank_small(uint32_t r, int n, const uint32_t* 
         out[0] = p & 0xffu;
         out[1] = p >> 8;
     } else if constexpr (J == 3) {
-        if (s_trip != nullptr) {
-            const uint32_t t = s_trip[r];
-            out[0] = t & 0xffu;
-            out[1] = (t >> 8) & 0xffu;
-            out[2] = t >> 16;
-        } else {
-            unrank_j(r, J, n, s_binom, Mimg, out);
+        int lo = 0, hi = n - 1;  // the last a with s_t3[a] <= r
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_t3[mid] <= r) lo = mid; else hi = mid - 1;
         }
+        const uint32_t a = static_cast<uint32_t>(lo);
+        const uint32_t p = s_pair[rank_pair_n(a, a, static_cast<uint32_t>(n)) + (r - s_t3[a])];
+        out[0] = a;
+        out[1] = p & 0xffu;
+        out[2] = p >> 8;
     } else if constexpr (J > 3) {
         unrank_j(r, J, n, s_binom, Mimg, out);
     }
@@ -162,15 +166,15 @@ __device__ __forceinline__ uint32_t rank_small(const uint32_t* t, int n, const u
 template <int K, int I>
 __device__ __forceinline__ void pull_stage(const uint64_t* __restrict__ in, uint64_t* __restrict__ dst, const uint32_t* cnt,
                                            int n, int Mimg, const uint32_t* s_binom, const uint16_t* s_pair,
-                                           const uint32_t* s_trip, const uint64_t* s_mask,
+                                           const uint32_t* s_t3, const uint64_t* s_mask,
                                            const uint64_t* __restrict__ won0, uint64_t full) {
     const uint32_t T = blockDim.x, tid = threadIdx.x;
     const uint32_t cb = cnt[K - I], cb_in = cnt[K - I + 1], rows = cnt[I] * cb;
     for (uint32_t o = tid; o < rows; o += T) {
         const uint32_t ra = o / cb, rb = o - ra * cb;
         uint32_t A[K], B[K + 1];
-        unrank_small<I>(ra, n, s_binom, Mimg, s_pair, s_trip, A);
-        unrank_small<K - I>(rb, n, s_binom, Mimg, s_pair, s_trip, B);
+        unrank_small<I>(ra, n, s_binom, Mimg, s_pair, s_t3, A);
+        unrank_small<K - I>(rb, n, s_binom, Mimg, s_pair, s_t3, B);
         const uint64_t row_in = static_cast<uint64_t>(rank_small<I - 1>(A, n, s_binom, Mimg)) * cb_in;
         uint64_t acc = 0, bits = s_mask[A[I - 1]];
         while (bits) {
@@ -211,7 +215,7 @@ __device__ __forceinline__ void pull_stage(const uint64_t* __restrict__ in, uint
 template <int K>
 __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ masks, uint8_t* __restrict__ tables,
                             BatchOut* __restrict__ out, uint64_t* __restrict__ scratch, uint64_t* s_mask,
-                            uint32_t* s_binom, unsigned long long* s_red, uint16_t* s_pair, uint32_t* s_trip) {
+                            uint32_t* s_binom, unsigned long long* s_red, uint16_t* s_pair, uint32_t* s_t3) {
     const int n = static_cast<int>(g.n), Mimg = n + K - 1;
     const uint32_t C = g.C, T = blockDim.x, tid = threadIdx.x;
     const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
@@ -226,29 +230,11 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
     uint32_t cnt[K + 1];
 #pragma unroll
     for (int j = 0; j <= K; ++j) cnt[j] = s_binom[j * (Mimg + 1) + (n + j - 1)];  // C(n+j-1, j)
-    // the triple table (K >= 3 and C(n+2, 3) <= kTripTab): each thread fills a
-    // contiguous run, one search for its first entry and lexicographic steps after
-    const uint32_t* trip = nullptr;
+    // the starts of the triples by first element (K >= 3)
+    const uint32_t* trip = s_t3;
     if constexpr (K >= 3) {
-        if (cnt[3] <= kTripTab) {
-            const uint32_t per = (cnt[3] + T - 1) / T, e0 = tid * per, e1 = min(cnt[3], e0 + per);
-            if (e0 < e1) {
-                uint32_t t3[3];
-                unrank_j(e0, 3, n, s_binom, Mimg, t3);
-                for (uint32_t e = e0; e < e1; ++e) {
-                    s_trip[e] = t3[0] | (t3[1] << 8) | (t3[2] << 16);
-                    if (t3[2] + 1 < static_cast<uint32_t>(n)) {
-                        ++t3[2];
-                    } else if (t3[1] + 1 < static_cast<uint32_t>(n)) {
-                        t3[1] = t3[2] = t3[1] + 1;
-                    } else {
-                        t3[0] = t3[1] = t3[2] = t3[0] + 1;
-                    }
-                }
-            }
-            __syncthreads();
-            trip = s_trip;
-        }
+        for (int a = tid; a < n; a += T) s_t3[a] = cnt[3] - small_binom(n - a + 2, 3);
+        __syncthreads();
     }
     uint64_t* won0 = scratch;
     uint64_t* won1 = won0 + C;
@@ -382,7 +368,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
     __shared__ uint32_t s_binom[(kMaxBK + 1) * (kBinomCols + 1)];
     __shared__ unsigned long long s_red[2 * kMaxBlockWaves];
     __shared__ uint16_t s_pair[kPairTab];
-    __shared__ uint32_t s_trip[kTripTab];
+    __shared__ uint32_t s_t3[kMaxBN];
     __shared__ uint32_t s_item;
     uint64_t* slot = scratch + static_cast<uint64_t>(blockIdx.x) * slot_words;
     for (;;) {
@@ -394,10 +380,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
         const uint32_t gi = order[item];
         const BatchGraph g = graphs[gi];
         switch (g.k) {
-            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_trip); break;
-            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_trip); break;
-            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_trip); break;
-            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_trip); break;
+            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3); break;
+            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3); break;
+            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3); break;
+            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3); break;
             default: break;
         }
         __syncthreads();
