@@ -343,10 +343,21 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
             best = key < best ? key : best;
         }
     best = block_min_all(best, s_red);
-    // order-independent table hash
+    // order-independent table hash, sum over bytes i of (b_i + 1) * mix64(i):
+    // 16 bytes per lane and step where the table is 16-byte aligned (tables
+    // start at 256-byte offsets of the table memory), then the tail bytes
     unsigned long long h = 0;
     const uint64_t bytes = 2ull * C * n;
-    for (uint64_t i = tid; i < bytes; i += T) h += (static_cast<uint64_t>(table[i]) + 1) * mix64(i);
+    const uint64_t chunks = (reinterpret_cast<uintptr_t>(table) & 15u) == 0 ? bytes / 16 : 0;
+    const uint4* table16 = reinterpret_cast<const uint4*>(table);
+    for (uint64_t j = tid; j < chunks; j += T) {
+        const uint4 v = table16[j];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            h += (static_cast<uint64_t>((w[t >> 2] >> (8 * (t & 3))) & 0xffu) + 1) * mix64(16 * j + t);
+    }
+    for (uint64_t i = 16 * chunks + tid; i < bytes; i += T) h += (static_cast<uint64_t>(table[i]) + 1) * mix64(i);
     unused = 0;
     block_sum_all(h, unused, s_red);
     if (tid == 0) {
