@@ -215,7 +215,8 @@ __device__ __forceinline__ void pull_stage(const uint64_t* __restrict__ in, uint
 template <int K>
 __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ masks, uint8_t* __restrict__ tables,
                             BatchOut* __restrict__ out, uint64_t* __restrict__ scratch, uint64_t* s_mask,
-                            uint32_t* s_binom, unsigned long long* s_red, uint16_t* s_pair, uint32_t* s_t3) {
+                            uint32_t* s_binom, unsigned long long* s_red, uint16_t* s_pair, uint32_t* s_t3,
+                            uint64_t* s_occ) {
     const int n = static_cast<int>(g.n), Mimg = n + K - 1;
     const uint32_t C = g.C, T = blockDim.x, tid = threadIdx.x;
     const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
@@ -245,25 +246,71 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
     uint8_t* last0 = reinterpret_cast<uint8_t*>(nx1 + C + 2ull * g.hmax);
     uint8_t* table = tables + g.table_off;
 
-    // captures: depth 0 in both columns (k_cops_visibility.cpp:85-127)
+    // captures: depth 0 in both columns (k_cops_visibility.cpp:85-127). The
+    // rows of a wave's 64 configurations are one contiguous span of the table
+    // (64 * 2n bytes from a 16-byte aligned start where the table is): lanes
+    // write consecutive 16-byte chunks of it, reading each row's cop mask
+    // from the wave's slot in s_occ. Otherwise each lane writes its row.
     unsigned long long base = 0, unused = 0;
-    for (uint32_t c = tid; c < C; c += T) {
-        uint32_t cops[K];
-        unrank_small<K>(c, n, s_binom, Mimg, s_pair, trip, cops);
+    const bool spans = (reinterpret_cast<uintptr_t>(table) & 15u) == 0;  // uniform
+    const uint32_t rowb = 2u * static_cast<uint32_t>(n), lane = tid & 63u;
+    uint64_t* occ_w = s_occ + (tid >> 6) * 64;
+    for (uint32_t c0 = 0; c0 < C; c0 += T) {
+        const uint32_t c = c0 + tid;
         uint64_t occ = 0;
+        if (c < C) {
+            uint32_t cops[K];
+            unrank_small<K>(c, n, s_binom, Mimg, s_pair, trip, cops);
 #pragma unroll
-        for (int j = 0; j < K; ++j) occ |= 1ull << cops[j];
-        won0[c] = occ;
-        won1[c] = occ;
-        fr1[c] = occ;
-        last0[c] = 0;
-        uint8_t* row = table + static_cast<uint64_t>(c) * n * 2;
-        for (int r = 0; r < n; ++r) {
-            const uint8_t b = ((occ >> r) & 1) ? won_byte(0) : kUnmarked;
-            row[2 * r] = b;
-            row[2 * r + 1] = b;
+            for (int j = 0; j < K; ++j) occ |= 1ull << cops[j];
+            won0[c] = occ;
+            won1[c] = occ;
+            fr1[c] = occ;
+            last0[c] = 0;
+            base += __builtin_popcountll(occ);
+            if (!spans) {
+                uint8_t* row = table + static_cast<uint64_t>(c) * rowb;
+                for (int r = 0; r < n; ++r) {
+                    const uint8_t b = ((occ >> r) & 1) ? won_byte(0) : kUnmarked;
+                    row[2 * r] = b;
+                    row[2 * r + 1] = b;
+                }
+            }
         }
-        base += __builtin_popcountll(occ);
+        if (spans) {
+            occ_w[lane] = occ;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t r0 = c0 + (tid & ~63u);  // the wave's first configuration
+            const uint32_t rows = r0 < C ? min(64u, C - r0) : 0u;
+            const uint32_t span = rows * rowb, full16 = span / 16;
+            uint8_t* dst = table + static_cast<uint64_t>(r0) * rowb;
+            for (uint32_t q = lane; q < full16; q += 64) {
+                uint32_t o = 16 * q, rl = o / rowb;
+                o -= rl * rowb;
+                uint64_t m = occ_w[rl];
+                uint32_t v[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    const uint32_t b = ((m >> (o >> 1)) & 1) ? won_byte(0) : kUnmarked;
+                    v[t >> 2] |= b << (8 * (t & 3));
+                    if (++o == rowb) {
+                        o = 0;
+                        ++rl;
+                        m = rl < 64 ? occ_w[rl] : 0ull;
+                    }
+                }
+                reinterpret_cast<uint4*>(dst)[q] = make_uint4(v[0], v[1], v[2], v[3]);
+            }
+            for (uint32_t i = 16 * full16 + lane; i < span; i += 64) {  // the last wave's tail bytes
+                const uint32_t rl = i / rowb, r = (i - rl * rowb) >> 1;
+                dst[i] = ((occ_w[rl] >> r) & 1) ? won_byte(0) : kUnmarked;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
     }
     block_sum_all(base, unused, s_red);
     if (tid == 0) {
@@ -380,6 +427,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
     __shared__ unsigned long long s_red[2 * kMaxBlockWaves];
     __shared__ uint16_t s_pair[kPairTab];
     __shared__ uint32_t s_t3[kMaxBN];
+    __shared__ uint64_t s_occ[kMaxBlockWaves * 64];  // per wave: the cop masks of its 64 configurations
     __shared__ uint32_t s_item;
     uint64_t* slot = scratch + static_cast<uint64_t>(blockIdx.x) * slot_words;
     for (;;) {
@@ -391,10 +439,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
         const uint32_t gi = order[item];
         const BatchGraph g = graphs[gi];
         switch (g.k) {
-            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3); break;
-            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3); break;
-            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3); break;
-            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3); break;
+            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ); break;
+            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ); break;
+            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ); break;
+            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ); break;
             default: break;
         }
         __syncthreads();
