@@ -216,11 +216,22 @@ template <int K>
 __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ masks, uint8_t* __restrict__ tables,
                             BatchOut* __restrict__ out, uint64_t* __restrict__ scratch, uint64_t* s_mask,
                             uint32_t* s_binom, unsigned long long* s_red, uint16_t* s_pair, uint32_t* s_t3,
-                            uint64_t* s_occ) {
+                            uint64_t* s_occ, uint64_t* s_nib) {
     const int n = static_cast<int>(g.n), Mimg = n + K - 1;
     const uint32_t C = g.C, T = blockDim.x, tid = threadIdx.x;
     const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
     for (int i = tid; i < n; i += T) s_mask[i] = masks[g.mask_off + i];
+    __syncthreads();  // s_mask, for the robber table
+    // robber-step table: s_nib[p * 16 + x] = the robbers with a move onto one
+    // of the vertices 4p + j for the set bits j of x
+    const int nnib = (n + 3) / 4;
+    for (int e = tid; e < nnib * 16; e += T) {
+        const uint64_t vs = static_cast<uint64_t>(e & 15) << (4 * (e >> 4));
+        uint64_t acc = 0;
+        for (int r = 0; r < n; ++r)
+            if (s_mask[r] & vs) acc |= 1ull << r;
+        s_nib[e] = acc;
+    }
     for (int i = tid; i < (K + 1) * (Mimg + 1); i += T) s_binom[i] = small_binom(i % (Mimg + 1), i / (Mimg + 1));
     // the pair table: row a of the lexicographic order holds (a, a..n-1)
     for (int a = 0; a < n; ++a) {
@@ -329,15 +340,26 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
             if constexpr (K >= 3) pull_stage<K, 3>(hbuf[1], K == 3 ? nx0 : hbuf[0], cnt, n, Mimg, s_binom, s_pair, trip, s_mask, won0, full);
             if constexpr (K >= 4) pull_stage<K, 4>(hbuf[0], nx0, cnt, n, Mimg, s_binom, s_pair, trip, s_mask, won0, full);
         }
-        // robber turn: every move of r lands on a won cop-turn state (depth <= d)
+        // robber turn: every move of r lands on a won cop-turn state (depth <= d).
+        // Each robber not won yet is tested against its moves; where a lane of
+        // the wave has more of them than nibbles, the wave takes the robbers
+        // with a move onto an open cop-turn state from the table instead (a
+        // lookup per nibble of the open set).
         if (do_rob) {
             for (uint32_t c = tid; c < C; c += T) {
                 const uint64_t w0 = won0[c];
                 uint64_t cand = full & ~won1[c], fresh = 0;
-                while (cand) {
-                    const int r = __builtin_ctzll(cand);
-                    cand &= cand - 1;
-                    if ((s_mask[r] & ~w0) == 0) fresh |= 1ull << r;
+                if (__ballot(__builtin_popcountll(cand) > nnib) != 0) {
+                    const uint64_t open = full & ~w0;
+                    uint64_t stay = 0;
+                    for (int p = 0; p < nnib; ++p) stay |= s_nib[p * 16 + ((open >> (4 * p)) & 15u)];
+                    fresh = cand & ~stay;
+                } else {
+                    while (cand) {
+                        const int r = __builtin_ctzll(cand);
+                        cand &= cand - 1;
+                        if ((s_mask[r] & ~w0) == 0) fresh |= 1ull << r;
+                    }
                 }
                 nx1[c] = fresh;
             }
@@ -428,6 +450,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
     __shared__ uint16_t s_pair[kPairTab];
     __shared__ uint32_t s_t3[kMaxBN];
     __shared__ uint64_t s_occ[kMaxBlockWaves * 64];  // per wave: the cop masks of its 64 configurations
+    __shared__ uint64_t s_nib[(kMaxBN + 3) / 4 * 16];  // robber step: robbers by nibble of target vertices
     __shared__ uint32_t s_item;
     uint64_t* slot = scratch + static_cast<uint64_t>(blockIdx.x) * slot_words;
     for (;;) {
@@ -439,10 +462,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
         const uint32_t gi = order[item];
         const BatchGraph g = graphs[gi];
         switch (g.k) {
-            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ); break;
-            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ); break;
-            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ); break;
-            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ); break;
+            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib); break;
+            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib); break;
+            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib); break;
+            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib); break;
             default: break;
         }
         __syncthreads();
