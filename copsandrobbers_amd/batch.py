@@ -12,7 +12,7 @@ from . import MAX_K, SolverError, _check, _u8, lib
 
 MAX_N = 64
 MAX_BATCH_K = 4
-TUNING = {"threads": 1, "blocks_per_cu": 2}
+TUNING = {"threads": 1, "blocks_per_cu": 2, "push_pct": 3}
 
 
 class GraphResult(ctypes.Structure):
@@ -206,7 +206,8 @@ class Batch:
         return out
 
     def set_tuning(self, **knobs):
-        """Launch shape (results never change): threads (0, 256, 512, 1024), blocks_per_cu."""
+        """Speed settings (results never change): threads (0, 256, 512, 1024), blocks_per_cu,
+        push_pct (0 = every level pulls)."""
         for name, value in knobs.items():
             _check(_lib().cr_batch_set_tuning(self._h, TUNING[name], int(value)))
 

@@ -67,7 +67,8 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     return x;
 }
 
-constexpr int kMaxBlockWaves = 16;  // workgroups of up to 1024 threads
+constexpr int kMaxBlockWaves = 16;
+constexpr int32_t kPushPct = 100;   // default push threshold (% of the pull's cost)  // workgroups of up to 1024 threads
 constexpr int kMlp = 8;             // row loads kept in flight per thread in the pull
 
 // Block-wide sums with the result in every thread.
@@ -216,7 +217,7 @@ template <int K>
 __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ masks, uint8_t* __restrict__ tables,
                             BatchOut* __restrict__ out, uint64_t* __restrict__ scratch, uint64_t* s_mask,
                             uint32_t* s_binom, unsigned long long* s_red, uint16_t* s_pair, uint32_t* s_t3,
-                            uint64_t* s_occ, uint64_t* s_nib) {
+                            uint64_t* s_occ, uint64_t* s_nib, uint8_t* s_inl, uint32_t push_pct) {
     const int n = static_cast<int>(g.n), Mimg = n + K - 1;
     const uint32_t C = g.C, T = blockDim.x, tid = threadIdx.x;
     const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
@@ -239,6 +240,14 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
         for (int b = a + static_cast<int>(tid); b < n; b += T) s_pair[base + (b - a)] = static_cast<uint16_t>(a | (b << 8));
     }
     __syncthreads();
+    // push: row v of s_inl holds v's in-degree, then the vertices with a move
+    // onto v (the single-vertex entries of the nibble table)
+    for (int v = tid; v < n; v += T) {
+        uint8_t* row = s_inl + v * (kMaxBN + 1);
+        uint32_t deg = 0;
+        for (uint64_t m = s_nib[(v >> 2) * 16 + (1 << (v & 3))]; m; m &= m - 1) row[1 + deg++] = static_cast<uint8_t>(__builtin_ctzll(m));
+        row[0] = static_cast<uint8_t>(deg);
+    }
     uint32_t cnt[K + 1];
 #pragma unroll
     for (int j = 0; j <= K; ++j) cnt[j] = s_binom[j * (Mimg + 1) + (n + j - 1)];  // C(n+j-1, j)
@@ -329,12 +338,71 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
         out->hist[0][1] = static_cast<uint32_t>(base);
         for (int d = 1; d < CR_MAX_ROUND; ++d) out->hist[d][0] = out->hist[d][1] = 0;
     }
+    // push and pull costs per level (row loads and relaxations)
+    uint32_t maxdeg = 0, edges = 0;
+    for (int v = 0; v < n; ++v) {
+        maxdeg = max(maxdeg, static_cast<uint32_t>(s_inl[v * (kMaxBN + 1)]));
+        edges += static_cast<uint32_t>(__builtin_popcountll(s_mask[v]));
+    }
+    unsigned long long push_unit = 1, pull_rows = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) push_unit *= maxdeg;
+#pragma unroll
+    for (int i = 1; i <= K; ++i) pull_rows += static_cast<unsigned long long>(cnt[i]) * cnt[K - i];
+    const unsigned long long pull_cost = pull_rows * edges / static_cast<unsigned long long>(n);
     unsigned long long won = 2 * base, prev0 = base, prev1 = base;
     uint32_t levels = 0;
     for (uint32_t d = 0; d + 1 < CR_MAX_ROUND; ++d) {
         const bool do_cop = prev1 != 0, do_rob = prev0 != 0;
         // cop turn: factorised pull from the robber frontier
-        if (do_cop) {
+        // a level with few robber-frontier states pushes: at most prev1 frontier
+        // rows times maxdeg^K predecessors against the pull's rows times the
+        // mean degree
+        const bool push = do_cop && push_pct != 0 && prev1 * push_unit * 100 <= static_cast<unsigned long long>(push_pct) * pull_cost;
+        if (do_cop && push) {
+            for (uint32_t c = tid; c < C; c += T) nx0[c] = 0;
+            __syncthreads();
+            // waves take 64 configurations at a time; the lanes enumerate the
+            // predecessors of each frontier row among them (duplicates only OR
+            // the same bits again)
+            for (uint32_t cb = (tid & ~63u); cb < C; cb += T) {
+                const uint32_t c = cb + lane;
+                const uint64_t f = c < C ? fr1[c] : 0ull;
+                for (uint64_t live = __ballot(f != 0); live; live &= live - 1) {
+                    const int j = __builtin_ctzll(live);
+                    const uint64_t bits = __shfl(f, j);
+                    uint32_t cops[K], deg[K];
+                    unrank_small<K>(cb + static_cast<uint32_t>(j), n, s_binom, Mimg, s_pair, trip, cops);
+                    uint32_t P = 1;
+#pragma unroll
+                    for (int i = 0; i < K; ++i) {
+                        deg[i] = s_inl[cops[i] * (kMaxBN + 1)];
+                        P *= deg[i];
+                    }
+                    for (uint32_t t = lane; t < P; t += 64) {
+                        uint32_t x[K], q = t;
+#pragma unroll
+                        for (int i = K - 1; i >= 0; --i) {
+                            const uint32_t qi = q / deg[i];
+                            x[i] = s_inl[cops[i] * (kMaxBN + 1) + 1 + (q - qi * deg[i])];
+                            q = qi;
+                        }
+#pragma unroll
+                        for (int i = 1; i < K; ++i)  // insertion sort
+#pragma unroll
+                            for (int m = i; m > 0; --m)
+                                if (x[m - 1] > x[m]) {
+                                    const uint32_t tmp = x[m - 1];
+                                    x[m - 1] = x[m];
+                                    x[m] = tmp;
+                                }
+                        atomicOr(reinterpret_cast<unsigned long long*>(nx0 + rank_small<K>(x, n, s_binom, Mimg)),
+                                 static_cast<unsigned long long>(bits));
+                    }
+                }
+            }
+            __syncthreads();
+        } else if (do_cop) {
             pull_stage<K, 1>(fr1, K == 1 ? nx0 : hbuf[0], cnt, n, Mimg, s_binom, s_pair, trip, s_mask, won0, full);
             if constexpr (K >= 2) pull_stage<K, 2>(hbuf[0], K == 2 ? nx0 : hbuf[1], cnt, n, Mimg, s_binom, s_pair, trip, s_mask, won0, full);
             if constexpr (K >= 3) pull_stage<K, 3>(hbuf[1], K == 3 ? nx0 : hbuf[0], cnt, n, Mimg, s_binom, s_pair, trip, s_mask, won0, full);
@@ -369,8 +437,13 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
         unsigned long long c0 = 0, c1 = 0;
         const uint8_t byte = won_byte(d + 1);
         for (uint32_t c = tid; c < C; c += T) {
-            const uint64_t a = do_cop ? nx0[c] : 0, b = do_rob ? nx1[c] : 0;
+            uint64_t a = do_cop ? nx0[c] : 0;
+            const uint64_t b = do_rob ? nx1[c] : 0;
             uint8_t* row = table + static_cast<uint64_t>(c) * n * 2;
+            if (a) {
+                const uint64_t w0 = won0[c];
+                a &= ~w0 & full;  // the push's candidates may include won states (the pull's are masked)
+            }
             if (a) {
                 won0[c] |= a;
                 last0[c] = static_cast<uint8_t>(d + 1);
@@ -443,7 +516,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
                                                         const uint32_t* __restrict__ order, uint32_t count,
                                                         const uint64_t* __restrict__ masks, uint8_t* tables,
                                                         BatchOut* out, uint64_t* scratch, uint64_t slot_words,
-                                                        unsigned int* next) {
+                                                        unsigned int* next, uint32_t push_pct) {
     __shared__ uint64_t s_mask[kMaxBN];
     __shared__ uint32_t s_binom[(kMaxBK + 1) * (kBinomCols + 1)];
     __shared__ unsigned long long s_red[2 * kMaxBlockWaves];
@@ -451,6 +524,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
     __shared__ uint32_t s_t3[kMaxBN];
     __shared__ uint64_t s_occ[kMaxBlockWaves * 64];  // per wave: the cop masks of its 64 configurations
     __shared__ uint64_t s_nib[(kMaxBN + 3) / 4 * 16];  // robber step: robbers by nibble of target vertices
+    __shared__ uint8_t s_inl[kMaxBN * (kMaxBN + 1)];  // push: per vertex, its in-degree and in-neighbours
     __shared__ uint32_t s_item;
     uint64_t* slot = scratch + static_cast<uint64_t>(blockIdx.x) * slot_words;
     for (;;) {
@@ -462,10 +536,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchGraph* __restr
         const uint32_t gi = order[item];
         const BatchGraph g = graphs[gi];
         switch (g.k) {
-            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib); break;
-            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib); break;
-            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib); break;
-            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib); break;
+            case 1: solve_graph<1>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib, s_inl, push_pct); break;
+            case 2: solve_graph<2>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib, s_inl, push_pct); break;
+            case 3: solve_graph<3>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib, s_inl, push_pct); break;
+            case 4: solve_graph<4>(g, masks, tables, out + gi, slot, s_mask, s_binom, s_red, s_pair, s_t3, s_occ, s_nib, s_inl, push_pct); break;
             default: break;
         }
         __syncthreads();
@@ -581,6 +655,7 @@ struct Engine {
     int cus = 1;
     int32_t threads = 0;        // threads per workgroup: 256, 512, 1024, or 0 = by the largest graph
     int32_t blocks_per_cu = 0;  // resident workgroups per CU (0 = 2048 threads per CU)
+    int32_t push_pct = kPushPct;  // a level pushes when its push costs at most push_pct % of the pull
     uint32_t block = kBlock;    // the last launch's workgroup size
     int32_t share = 1;          // launches in flight together (server slots): each gets 1/share of the CUs
 
@@ -661,7 +736,8 @@ struct Engine {
         auto args = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, d_graphs.as<BatchGraph>(),
                                d_order.as<uint32_t>(), count, d_masks.as<uint64_t>(), d_tables.as<uint8_t>(),
-                               d_out.as<BatchOut>(), d_scratch.as<uint64_t>(), slot_words, d_next.as<unsigned int>());
+                               d_out.as<BatchOut>(), d_scratch.as<uint64_t>(), slot_words, d_next.as<unsigned int>(),
+                               static_cast<uint32_t>(push_pct));
         };
         if (block == 1024) args(k_batch_solve<1024>);
         else if (block == 512) args(k_batch_solve<512>);
@@ -960,6 +1036,10 @@ int set_engine_tuning(Engine& e, int32_t key, int64_t value) {
         case CR_BATCH_TUNE_BLOCKS_PER_CU:
             if (value < 0 || value > 32) return fail(CR_ERR_ARG, "workgroups per CU must be 0..32");
             e.blocks_per_cu = static_cast<int32_t>(value);
+            return CR_OK;
+        case CR_BATCH_TUNE_PUSH_PCT:
+            if (value < 0 || value > 1000000000) return fail(CR_ERR_ARG, "push threshold must be 0..10^9 %%");
+            e.push_pct = static_cast<int32_t>(value);
             return CR_OK;
         default: return fail(CR_ERR_ARG, "unknown batch tuning key %d", key);
     }
@@ -1306,7 +1386,7 @@ extern "C" int cr_batch_solve_noalloc(int device, int32_t count, const int32_t* 
         uint64_t* slots = reinterpret_cast<uint64_t*>(base + kOffSlots);
         auto go = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(grid)), dim3(block), 0, nullptr, ds->graphs, ds->order,
-                               nvalid, ds->masks, d_tables, dout, slots, slot_words, ds->next);
+                               nvalid, ds->masks, d_tables, dout, slots, slot_words, ds->next, kPushPct);
         };
         if (block == 1024) go(k_batch_solve<1024>);
         else go(k_batch_solve<256>);

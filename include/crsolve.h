@@ -346,7 +346,11 @@ int cr_batch_solve(cr_batch* b, int32_t count, const int32_t* n, const int32_t* 
 enum cr_batch_tuning {
     CR_BATCH_TUNE_THREADS = 1,      /* threads per workgroup (one graph at a time): 256, 512, 1024; 0 = 1024
                                        when the batch's largest graph has >= 4096 configurations, else 256 */
-    CR_BATCH_TUNE_BLOCKS_PER_CU = 2 /* persistent workgroups per CU; 0 = 2048 threads per CU */
+    CR_BATCH_TUNE_BLOCKS_PER_CU = 2, /* persistent workgroups per CU; 0 = 2048 threads per CU */
+    CR_BATCH_TUNE_PUSH_PCT = 3       /* a level's cop turn pushes from its robber frontier instead of
+                                        pulling when the push's bound (frontier states x largest
+                                        in-degree^k) is at most this % of the pull's row loads (default
+                                        100; 0 = always pull) */
 };
 int cr_batch_set_tuning(cr_batch* b, int32_t key, int64_t value);
 /* Per-pass "Found n" counts of graph `index` of the last cr_batch_solve. */

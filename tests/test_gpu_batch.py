@@ -152,6 +152,30 @@ def test_batch_matches_single_graph_solves(batch, solver, oracle):
             assert r["plys"] == verdict["plys"] and r["positions"] == verdict["positions"], i
 
 
+@pytest.mark.parametrize("push_pct", [0, 10**9], ids=["always_pull", "always_push"])
+def test_push_and_pull_levels_give_the_same_tables(batch, oracle, push_pct):
+    """Every level pulled, or every level's cop turn pushed from its robber
+    frontier (predecessors enumerated from in-neighbour lists), gives the
+    tables of the default settings (which push only the sparse levels), and
+    the oracle's on the directed, stay-move, n = 64 and k = 4 graphs."""
+    graphs = random_graphs(24, seed_base=7100)
+    say("batch push", push_pct, len(graphs))
+    _, ref = batch.solve(graphs, tables=True)
+    try:
+        batch.set_tuning(push_pct=push_pct)
+        res, tabs = batch.solve(graphs, tables=True)
+        counts = [batch.pass_counts(i) for i in range(len(graphs))]
+    finally:
+        batch.set_tuning(push_pct=100)
+    for i, (adj, k) in enumerate(graphs):
+        assert res[i]["status"] == 0, i
+        assert np.array_equal(tabs[i], ref[i]), (push_pct, i)
+    for i, (adj, k) in enumerate(graphs[-4:], start=len(graphs) - 4):  # directed, stay moves, n = 64, k = 4
+        want = oracle.v1_solve(adj, k, 1)
+        assert np.array_equal(tabs[i], want.states), (push_pct, i)
+        assert counts[i] == want.pass_counts, (push_pct, i)
+
+
 def test_benchmarked_batch_equals_single_solves_and_oracle(batch, solver, oracle):
     """The 4096 graphs the batch benchmark solves (workloads.batch_graphs(4096),
     one packed launch): every table's hash, pass count, verdict and capture
