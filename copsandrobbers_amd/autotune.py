@@ -45,6 +45,7 @@ PLAIN_SPACE = [
 BATCH_SPACE = [
     ("threads", [0, 256, 512, 1024]),
     ("blocks_per_cu", [0, 1, 2, 4, 8]),
+    ("push_pct", [0, 30, 100, 300]),
 ]
 
 
@@ -147,7 +148,7 @@ def _apply_plain(solver, settings):
 PLAIN_DEFAULTS = {"pull_scale": -1, "pull_gate": 4, "pull_pair": 1, "pull_tile_rows": 4, "pull_tile1": 9,
                   "pull_tile_chunk": 16, "pull_tile_blocks": 8192, "pair_blocks": 1024, "robber_open": 2,
                   "levels_per_sync": 8}
-BATCH_DEFAULTS = {"threads": 0, "blocks_per_cu": 0}
+BATCH_DEFAULTS = {"threads": 0, "blocks_per_cu": 0, "push_pct": 100}
 
 
 def tune_batch(batch, graphs, reps=3, space=BATCH_SPACE, rounds=2, min_gain=0.02, log=None):
