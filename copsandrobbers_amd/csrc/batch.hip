@@ -177,6 +177,21 @@ __device__ __forceinline__ void pull_stage(const uint64_t* __restrict__ in, uint
         unrank_small<I>(ra, n, s_binom, Mimg, s_pair, s_t3, A);
         unrank_small<K - I>(rb, n, s_binom, Mimg, s_pair, s_t3, B);
         const uint64_t row_in = static_cast<uint64_t>(rank_small<I - 1>(A, n, s_binom, Mimg)) * cb_in;
+        // rank terms of B for its (J+1)-multisets B + v (J = K - I): with T(p, x)
+        // the term of value x at position p, rank(B + v) = rank0 - sum_{B_t < v}
+        // T(t, B_t) - T(q, v) - sum_{B_t >= v} T(t + 1, B_t), q = #{B_t < v}
+        constexpr int J = K - I, J1 = J + 1;
+        const int Mp = Mimg + 1, M1 = n + J;
+        uint32_t U[J > 0 ? J : 1], S[J > 0 ? J : 1];
+        uint32_t rank0 = 0;
+        if constexpr (J1 >= 3) {
+            rank0 = s_binom[J1 * Mp + M1] - 1;
+#pragma unroll
+            for (int t = 0; t < J; ++t) {
+                U[t] = s_binom[(J1 - t) * Mp + (M1 - 1 - static_cast<int>(B[t]) - t)];
+                S[t] = s_binom[(J - t) * Mp + (M1 - 2 - static_cast<int>(B[t]) - t)];
+            }
+        }
         uint64_t acc = 0, bits = s_mask[A[I - 1]];
         while (bits) {
             // ranks of up to kMlp successor rows first, then their loads together
@@ -187,19 +202,31 @@ __device__ __forceinline__ void pull_stage(const uint64_t* __restrict__ in, uint
                 if (bits == 0) break;
                 const uint32_t v = static_cast<uint32_t>(__builtin_ctzll(bits));
                 bits &= bits - 1;
-                uint32_t M2[K + 1];
-                int pos = 0;
-                bool placed = false;
+                if constexpr (J1 >= 3) {
+                    uint32_t sum = 0;
+                    int q = 0;
 #pragma unroll
-                for (int t = 0; t < K - I; ++t) {
-                    if (!placed && v <= B[t]) {
-                        M2[pos++] = v;
-                        placed = true;
+                    for (int t = 0; t < J; ++t) {
+                        const bool lt = B[t] < v;
+                        sum += lt ? U[t] : S[t];
+                        q += lt ? 1 : 0;
                     }
-                    M2[pos++] = B[t];
+                    idx[u] = rank0 - sum - s_binom[(J1 - q) * Mp + (M1 - 1 - static_cast<int>(v) - q)];
+                } else {
+                    uint32_t M2[K + 1];
+                    int pos = 0;
+                    bool placed = false;
+#pragma unroll
+                    for (int t = 0; t < K - I; ++t) {
+                        if (!placed && v <= B[t]) {
+                            M2[pos++] = v;
+                            placed = true;
+                        }
+                        M2[pos++] = B[t];
+                    }
+                    if (!placed) M2[pos++] = v;
+                    idx[u] = rank_small<K - I + 1>(M2, n, s_binom, Mimg);
                 }
-                if (!placed) M2[pos++] = v;
-                idx[u] = rank_small<K - I + 1>(M2, n, s_binom, Mimg);
                 m = u + 1;
             }
             uint64_t got[kMlp];
