@@ -463,16 +463,27 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
         // finalize: depth d+1
         unsigned long long c0 = 0, c1 = 0;
         const uint8_t byte = won_byte(d + 1);
+        // the next configuration's rows are loaded while this one's bytes are stored
+        uint64_t a_nx = 0, b_nx = 0, w0_nx = 0, w1_nx = 0;
+        if (tid < C) {
+            a_nx = do_cop ? nx0[tid] : 0;
+            b_nx = do_rob ? nx1[tid] : 0;
+            w0_nx = do_cop ? won0[tid] : 0;
+            w1_nx = do_rob ? won1[tid] : 0;
+        }
         for (uint32_t c = tid; c < C; c += T) {
-            uint64_t a = do_cop ? nx0[c] : 0;
-            const uint64_t b = do_rob ? nx1[c] : 0;
-            uint8_t* row = table + static_cast<uint64_t>(c) * n * 2;
-            if (a) {
-                const uint64_t w0 = won0[c];
-                a &= ~w0 & full;  // the push's candidates may include won states (the pull's are masked)
+            uint64_t a = a_nx;
+            const uint64_t b = b_nx, w0 = w0_nx, w1 = w1_nx;
+            if (c + T < C) {
+                a_nx = do_cop ? nx0[c + T] : 0;
+                b_nx = do_rob ? nx1[c + T] : 0;
+                w0_nx = do_cop ? won0[c + T] : 0;
+                w1_nx = do_rob ? won1[c + T] : 0;
             }
+            uint8_t* row = table + static_cast<uint64_t>(c) * n * 2;
+            a &= ~w0 & full;  // the push's candidates may include won states (the pull's are masked)
             if (a) {
-                won0[c] |= a;
+                won0[c] = w0 | a;
                 last0[c] = static_cast<uint8_t>(d + 1);
                 uint64_t m = a;
                 while (m) {
@@ -482,7 +493,7 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
                 c0 += __builtin_popcountll(a);
             }
             if (b) {
-                won1[c] |= b;
+                won1[c] = w1 | b;
                 uint64_t m = b;
                 while (m) {
                     row[2 * __builtin_ctzll(m) + 1] = byte;
