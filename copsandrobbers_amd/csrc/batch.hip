@@ -441,9 +441,14 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
         // with a move onto an open cop-turn state from the table instead (a
         // lookup per nibble of the open set).
         if (do_rob) {
+            uint64_t w0_nx = tid < C ? won0[tid] : 0, w1_nx = tid < C ? won1[tid] : 0;  // loaded a row ahead
             for (uint32_t c = tid; c < C; c += T) {
-                const uint64_t w0 = won0[c];
-                uint64_t cand = full & ~won1[c], fresh = 0;
+                const uint64_t w0 = w0_nx, w1 = w1_nx;
+                if (c + T < C) {
+                    w0_nx = won0[c + T];
+                    w1_nx = won1[c + T];
+                }
+                uint64_t cand = full & ~w1, fresh = 0;
                 if (__ballot(__builtin_popcountll(cand) > nnib) != 0) {
                     const uint64_t open = full & ~w0;
                     uint64_t stay = 0;
