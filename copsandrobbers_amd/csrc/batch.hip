@@ -381,10 +381,9 @@ __device__ void solve_graph(const BatchGraph& g, const uint64_t* __restrict__ ma
     uint32_t levels = 0;
     for (uint32_t d = 0; d + 1 < CR_MAX_ROUND; ++d) {
         const bool do_cop = prev1 != 0, do_rob = prev0 != 0;
-        // cop turn: factorised pull from the robber frontier
-        // a level with few robber-frontier states pushes: at most prev1 frontier
-        // rows times maxdeg^K predecessors against the pull's rows times the
-        // mean degree
+        // cop turn: the factorised pull from the robber frontier, or a push from
+        // it where the frontier is small: at most prev1 frontier rows times
+        // maxdeg^K predecessors against the pull's rows times the mean degree
         const bool push = do_cop && push_pct != 0 && prev1 * push_unit * 100 <= static_cast<unsigned long long>(push_pct) * pull_cost;
         if (do_cop && push) {
             for (uint32_t c = tid; c < C; c += T) nx0[c] = 0;
