@@ -181,6 +181,40 @@ __device__ __forceinline__ uint32_t choose_pull(uint32_t frontier_rows, uint32_t
     return frontier_rows > threshold ? 1u : 0u;
 }
 
+// Loads / stores of one W-word row: 16-byte vector accesses where the row is
+// 16-byte aligned (W = 2, 4).
+template <int W>
+__device__ __forceinline__ void load_row(const uint64_t* __restrict__ src, uint64_t* v) {
+    if constexpr (W == 4) {
+        const ulonglong2 x = reinterpret_cast<const ulonglong2*>(src)[0];
+        const ulonglong2 y = reinterpret_cast<const ulonglong2*>(src)[1];
+        v[0] = x.x;
+        v[1] = x.y;
+        v[2] = y.x;
+        v[3] = y.y;
+    } else if constexpr (W == 2) {
+        const ulonglong2 x = reinterpret_cast<const ulonglong2*>(src)[0];
+        v[0] = x.x;
+        v[1] = x.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < W; ++q) v[q] = src[q];
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void store_row(uint64_t* __restrict__ dst, const uint64_t* v) {
+    if constexpr (W == 4) {
+        reinterpret_cast<ulonglong2*>(dst)[0] = make_ulonglong2(v[0], v[1]);
+        reinterpret_cast<ulonglong2*>(dst)[1] = make_ulonglong2(v[2], v[3]);
+    } else if constexpr (W == 2) {
+        reinterpret_cast<ulonglong2*>(dst)[0] = make_ulonglong2(v[0], v[1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < W; ++q) dst[q] = v[q];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // init: reset every state, seed captures (k_cops_visibility.cpp:85-127) and
 // resolve depth 1 in closed form. With no stay moves and no vertex without a
@@ -277,22 +311,26 @@ __global__ __launch_bounds__(kBlock) void k_init(GraphView gv, LevelArgs a) {
                 const uint32_t b1 = cap ? 0x01u : ((r1w & bit) ? 0x03u : kUnmarked);
                 return b0 | (b1 << 8);
             };
-            if (live) {
+            if (live) {  // the rows' bitsets by 16-byte stores where the rows allow
                 const uint64_t o = static_cast<uint64_t>(row) * W;
+                uint64_t v0[W], v1[W], z[W];
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
-                    a.won0[o + q] = cm[q] | c1[q];
-                    a.won1[o + q] = cm[q] | r1[q];
-                    a.fr1s[1][o + q] = r1[q];  // depth 1, read by the cop step of level 1
-                    if (a.fr1s[0] != a.fr1s[1]) a.fr1s[0][o + q] = 0;
-                    a.fr0[o + q] = c1[q];
-                    a.nx0[o + q] = 0;
+                    v0[q] = cm[q] | c1[q];
+                    v1[q] = cm[q] | r1[q];
+                    z[q] = 0;
                     lc += __popcll(cm[q]);
                     l0 += __popcll(c1[q]);
                     l1 += __popcll(r1[q]);
                     has0 |= c1[q] != 0;
                     has1 |= r1[q] != 0;
                 }
+                store_row<W>(a.won0 + o, v0);
+                store_row<W>(a.won1 + o, v1);
+                store_row<W>(a.fr1s[1] + o, r1);  // depth 1, read by the cop step of level 1
+                if (a.fr1s[0] != a.fr1s[1]) store_row<W>(a.fr1s[0] + o, z);
+                store_row<W>(a.fr0 + o, c1);
+                store_row<W>(a.nx0 + o, z);
             }
             if ((n & 7) == 0) {
                 // the wave's rows are one contiguous span of 64 * 2n bytes: lanes
@@ -498,40 +536,6 @@ struct SepStage {
 // The stage's input rows: stage 1 reads the robber frontier of the level's depth.
 __device__ __forceinline__ const uint64_t* stage_input(const SepStage& st, const Ctl* ctl) {
     return st.in_odd && (ctl->level & 1) ? st.in_odd : st.in;
-}
-
-// Loads / stores of one W-word row: 16-byte vector accesses where the row is
-// 16-byte aligned (W = 2, 4).
-template <int W>
-__device__ __forceinline__ void load_row(const uint64_t* __restrict__ src, uint64_t* v) {
-    if constexpr (W == 4) {
-        const ulonglong2 x = reinterpret_cast<const ulonglong2*>(src)[0];
-        const ulonglong2 y = reinterpret_cast<const ulonglong2*>(src)[1];
-        v[0] = x.x;
-        v[1] = x.y;
-        v[2] = y.x;
-        v[3] = y.y;
-    } else if constexpr (W == 2) {
-        const ulonglong2 x = reinterpret_cast<const ulonglong2*>(src)[0];
-        v[0] = x.x;
-        v[1] = x.y;
-    } else {
-#pragma unroll
-        for (int q = 0; q < W; ++q) v[q] = src[q];
-    }
-}
-
-template <int W>
-__device__ __forceinline__ void store_row(uint64_t* __restrict__ dst, const uint64_t* v) {
-    if constexpr (W == 4) {
-        reinterpret_cast<ulonglong2*>(dst)[0] = make_ulonglong2(v[0], v[1]);
-        reinterpret_cast<ulonglong2*>(dst)[1] = make_ulonglong2(v[2], v[3]);
-    } else if constexpr (W == 2) {
-        reinterpret_cast<ulonglong2*>(dst)[0] = make_ulonglong2(v[0], v[1]);
-    } else {
-#pragma unroll
-        for (int q = 0; q < W; ++q) dst[q] = v[q];
-    }
 }
 
 constexpr int kPullMlp = 8;  // input rows whose loads a thread keeps in flight
