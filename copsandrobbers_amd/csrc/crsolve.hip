@@ -1565,9 +1565,9 @@ __global__ __launch_bounds__(kBlock) void k_cop_finalize(GraphView gv, LevelArgs
                 if (pop) store_row<W>(a.won0 + o, wv);
                 store_row<W>(a.nx0 + o, z);
                 if (pop) {  // a listed row: its new states for the robber step and k_log_lists
+                    store_row<W>(a.fr0 + o, nw);
 #pragma unroll
                     for (int q = 0; q < W; ++q) {
-                        a.fr0[o + q] = nw[q];
                         if (a.log_rows) continue;  // the depth bytes go through the log
                         uint64_t bits = nw[q];
                         while (bits) {
