@@ -13,6 +13,8 @@
   the reference's), on 10^6 random states and every state of 1,000 random
   configuration rows; the base captures equal their closed form; the pass
   counts and the verdict are re-derived from a full scan of the copied table.
+  A second solve through other code paths (separate finalize, a small depth
+  log, a tiled level scan) gives the same table byte for byte.
 """
 import sys
 
@@ -73,18 +75,25 @@ def test_er200_k3_table_equals_oracle_port(oracle, er_k3, setting):
         assert verdict["best_cid"] == best and verdict["plys"] == plys
 
 
-def test_planar200_k4_table_satisfies_the_reference_equations(oracle):
+@pytest.fixture(scope="module")
+def planar_k4(oracle):
+    """The default-setting device solve of planar200 k = 4, its table copied to the host."""
     path = matrix("planar200_e400_s1")
-    n, adj = oracle.load_matrix(path)
     k = 4
-    C = oracle.config_count(n, k)
-    S = C * n * 2
     say("planar200 k=4 on the device")
     with cra.Solver(0) as s:
         base, passes, verdict = solve(s, path, k)
         dev_levels = s.level_counts()
-        say("copy table to host:", S, "bytes")
+        say("copy table to host:", s.info().num_states, "bytes")
         states = s.states()
+    return path, k, base, passes, verdict, dev_levels, states
+
+
+def test_planar200_k4_table_satisfies_the_reference_equations(oracle, planar_k4):
+    path, k, base, passes, verdict, dev_levels, states = planar_k4
+    n, adj = oracle.load_matrix(path)
+    C = oracle.config_count(n, k)
+    S = C * n * 2
     assert states.size == S
     rng = np.random.default_rng(2026)
     ids = [rng.integers(0, S, 1_000_000, dtype=np.uint64)]
@@ -110,3 +119,28 @@ def test_planar200_k4_table_satisfies_the_reference_equations(oracle):
     # the table's depths must fit under the cap and every depth level be contiguous
     depths = np.flatnonzero(levels.sum(axis=1))
     assert depths.max() <= 126 and np.array_equal(depths, np.arange(depths.max() + 1))
+
+
+# Non-default paths at this size: the separate finalize after a cop step that
+# runs first, a depth log too small for a chunk of levels (applied at every
+# synchronisation, and depths that do not fit write their bytes directly), and
+# the two-level level scan with 512-group tiles.
+PLANAR_OTHER_PATHS = {"fuse_finalize": 0, "depth_log": 4, "scan_tile": 512}
+
+
+def test_planar200_k4_table_is_the_same_on_other_paths(planar_k4):
+    path, k, base, passes, verdict, dev_levels, want = planar_k4
+    say("planar200 k=4 on the device", PLANAR_OTHER_PATHS)
+    with cra.Solver(0) as s:
+        s.set_tuning(**PLANAR_OTHER_PATHS)
+        got_base, got_passes, got_verdict = solve(s, path, k)
+        assert (got_base, got_passes, got_verdict) == (base, passes, verdict)
+        assert np.array_equal(s.level_counts(), dev_levels)
+        step = 1 << 30
+        say("compare the table in slices of", step, "bytes")
+        for off in range(0, want.size, step):
+            cnt = min(step, want.size - off)
+            got = s.states(off, cnt)
+            if not np.array_equal(got, want[off:off + cnt]):
+                bad = np.flatnonzero(got != want[off:off + cnt])
+                pytest.fail(f"{bad.size} states differ in [{off}, {off + cnt}), the first at {off + int(bad[0])}")
