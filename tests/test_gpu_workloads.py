@@ -13,8 +13,9 @@
   the reference's), on 10^6 random states and every state of 1,000 random
   configuration rows; the base captures equal their closed form; the pass
   counts and the verdict are re-derived from a full scan of the copied table.
-  A second solve through other code paths (separate finalize, a small depth
-  log, a tiled level scan) gives the same table byte for byte.
+  Solves through other code paths (separate finalize, a small depth log, a
+  tiled level scan; other tile shapes, robber-step and push variants) give the
+  same table byte for byte.
 """
 import sys
 
@@ -121,18 +122,23 @@ def test_planar200_k4_table_satisfies_the_reference_equations(oracle, planar_k4)
     assert depths.max() <= 126 and np.array_equal(depths, np.arange(depths.max() + 1))
 
 
-# Non-default paths at this size: the separate finalize after a cop step that
-# runs first, a depth log too small for a chunk of levels (applied at every
-# synchronisation, and depths that do not fit write their bytes directly), and
-# the two-level level scan with 512-group tiles.
-PLANAR_OTHER_PATHS = {"fuse_finalize": 0, "depth_log": 4, "scan_tile": 512}
+# Non-default paths at this size: (1) the separate finalize after a cop step
+# that runs first, a depth log too small for a chunk of levels (applied at every
+# synchronisation, and depths that do not fit write their bytes directly), the
+# two-level level scan with 512-group tiles; (2) the first pull stage through
+# k_pull_tile, 2-row tiles, the robber step from the new cop-turn states, a
+# push that marks no groups, and a higher pull threshold (more push levels).
+PLANAR_OTHER_PATHS = [{"fuse_finalize": 0, "depth_log": 4, "scan_tile": 512},
+                      {"pull_tile1": 0, "pull_tile_rows": 2, "robber_open": 0, "push_dirty": 0, "pull_scale": 400}]
+PLANAR_OTHER_IDS = ["finalize_small_log_tiled_scan", "tile_shapes_robber_by_new_more_pushes"]
 
 
-def test_planar200_k4_table_is_the_same_on_other_paths(planar_k4):
+@pytest.mark.parametrize("setting", PLANAR_OTHER_PATHS, ids=PLANAR_OTHER_IDS)
+def test_planar200_k4_table_is_the_same_on_other_paths(planar_k4, setting):
     path, k, base, passes, verdict, dev_levels, want = planar_k4
-    say("planar200 k=4 on the device", PLANAR_OTHER_PATHS)
+    say("planar200 k=4 on the device", setting)
     with cra.Solver(0) as s:
-        s.set_tuning(**PLANAR_OTHER_PATHS)
+        s.set_tuning(**setting)
         got_base, got_passes, got_verdict = solve(s, path, k)
         assert (got_base, got_passes, got_verdict) == (base, passes, verdict)
         assert np.array_equal(s.level_counts(), dev_levels)
